@@ -1,0 +1,216 @@
+"""Benchmark of the MI355X BVH pipeline on the ~4M-triangle scene.
+
+Workload (one GPU): 2,048 seeded icospheres (subdivision 2-5) + 500,000
+soup triangles (4.24M triangles, fp32), 63-bit Morton LBVH build + refit;
+then 1920x1080 primary rays + one cosine-weighted diffuse bounce; then a
+short plain Bittner13 optimisation (1% of the nodes per step).
+
+A "step" is one LBVH build of the whole scene, timed like LBVH::runImpl
+(lib/builder/LBVH.cpp:41-50: encode -> sort -> leaves/topology -> refit ->
+sync) with the mesh already resident in HBM.  `value` is Mtri/s of the
+build.  Per-kernel durations come from HIP events on the library's stream;
+the dominant kernel's algorithmic bytes per launch over its mean duration
+give the roofline figure.  The CPU baseline is the oracle's OpenMP LBVH
+(oracle/oracle_baseline.cpp) on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LBVH Mtri/s + Mrays/s; time to reach plain-Bittner SAH (learned vs plain)"
+HBM_PEAK_GBS = 8000.0
+CAM = dict(pos=(0.5, 0.5, 2.0), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov=45.0)
+
+
+def kernel_bytes(name, n, morton_bits, pass_idx, passes):
+    """Algorithmic HBM bytes of one launch, per the byte model of the build:
+    scene box: face 12 + 3 vertex gathers counted once per vertex (12 V/n);
+    Morton: face 12 + vertices 12 V/n + key write; onesweep pass: key + value
+    read and written (first pass reads no value, last pass writes no key);
+    level-0 build: sorted index 4 + face 12 + vertices, leaf 48 + inner 48
+    written per triangle."""
+    kb = 8 if morton_bits == 63 else 4
+    if name == "k_onesweep":
+        rd = kb + (0 if pass_idx == 0 else 4)
+        wr = (0 if pass_idx == passes - 1 else kb) + 4
+        return n * (rd + wr)
+    return None
+
+
+def run_bench(args):
+    import bvhtest_amd as bt
+
+    L = bt.lib()
+    if L.bvht_deviceCount() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    t0 = time.time()
+    v, f = bt.scene_spheres_soup(2048, 500000, 2)
+    n = len(f)
+    print(f"# scene: {n} triangles, {len(v)} vertices ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    mesh = bt.DeviceMesh(v, f)
+    bits = args.morton_bits
+    builder = bt.LBVHBuilder(mesh, bits)
+    tree = builder.allocate_bvh()
+
+    # warmup
+    for _ in range(args.warmup):
+        builder.build(tree)
+    L.LBVH_doCUDASyc()
+    # timed region: K builds, stream-synchronised on both sides
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        builder.build(tree)
+    L.LBVH_doCUDASyc()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    mtri = n * args.steps / wall / 1e6
+    print(f"# build: {ms_per_step:.3f} ms/step, {mtri:.1f} Mtri/s", file=sys.stderr, flush=True)
+
+    # per-kernel timing (separate, instrumented builds)
+    L.bvht_LBVH_setTiming(True)
+    per = {}
+    reps = max(3, min(args.steps, 10))
+    for _ in range(reps):
+        builder.build(tree)
+        buf = (ctypes.c_float * 32)()
+        k = L.bvht_LBVH_stageTimes(buf, 32)
+        pass_idx = 0
+        for i in range(k):
+            name = L.bvht_LBVH_stageName(i).decode()
+            key = name
+            if name == "k_onesweep":
+                key = f"k_onesweep[{pass_idx}]"
+                pass_idx += 1
+            per.setdefault(key, []).append(buf[i])
+    L.bvht_LBVH_setTiming(False)
+    passes = sum(1 for k in per if k.startswith("k_onesweep"))
+    stages = {k: float(np.mean(x)) for k, x in per.items()}
+    # dominant kernel family: the onesweep passes
+    sort_ms = sum(vv for kk, vv in stages.items() if kk.startswith("k_onesweep"))
+    sort_bytes = sum(kernel_bytes("k_onesweep", n, bits, p, passes) for p in range(passes))
+    dom_ms = sort_ms / passes
+    dom_bytes = sort_bytes / passes
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_onesweep (mean of %d passes)" % passes, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": int(dom_bytes), "ms_per_launch": round(dom_ms, 4)}
+    print(f"# stages (ms): {json.dumps({k: round(x, 4) for k, x in stages.items()})}", file=sys.stderr, flush=True)
+
+    # primary + bounce rays
+    w, h = 1920, 1080
+    nrays = w * h
+    rays_dev = bt.generate_rays_device(w, h, **CAM)
+    hits_dev = bt.DeviceBuffer(nrays * bt.HIT_DTYPE.itemsize)
+    img_dev = bt.DeviceBuffer(nrays * bt.PIXEL_DTYPE.itemsize)
+    brays_dev = bt.DeviceBuffer(nrays * bt.RAY_DTYPE.itemsize)
+    skip_dev = bt.DeviceBuffer(nrays * 4)
+    bhits_dev = bt.DeviceBuffer(nrays * bt.HIT_DTYPE.itemsize)
+    light = bt.vec3((2.0, 2.0, 2.0))
+
+    def trace_once():
+        L.generateRays(rays_dev.ptr, w, h, bt.vec3(CAM["pos"]), bt.vec3(CAM["look_at"]), bt.vec3(CAM["up"]), CAM["fov"])
+        L.bvht_traceRays(rays_dev.ptr, nrays, tree.ptr.nodes, 0, mesh.raw, light, 0, 0.0, None, hits_dev.ptr, img_dev.ptr)
+        L.bvht_makeBounceRays(rays_dev.ptr, hits_dev.ptr, nrays, mesh.raw, 1, brays_dev.ptr, skip_dev.ptr)
+        L.bvht_traceRays(brays_dev.ptr, nrays, tree.ptr.nodes, 0, mesh.raw, light, 1, 0.0, skip_dev.ptr, bhits_dev.ptr,
+                         None)
+
+    trace_once()
+    L.tracerDoCudaSync()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        trace_once()
+    L.tracerDoCudaSync()
+    trace_s = (time.perf_counter() - t0) / reps
+    hits = hits_dev.download(bt.HIT_DTYPE, nrays)
+    bhits = bhits_dev.download(bt.HIT_DTYPE, nrays)
+    mrays = 2 * nrays / trace_s / 1e6
+    trace = {"mrays_per_s": round(mrays, 1), "primary_rays": nrays, "bounce_rays": nrays,
+             "ms_per_frame": round(trace_s * 1e3, 3),
+             "primary_hit_frac": round(float((hits["triId"] != 0xFFFFFFFF).mean()), 4),
+             "bounce_hit_frac": round(float((bhits["triId"] != 0xFFFFFFFF).mean()), 4)}
+    print(f"# trace: {trace}", file=sys.stderr, flush=True)
+
+    # plain Bittner13 (1% of the nodes per step), bounded number of steps
+    bittner = None
+    if args.bittner_steps > 0:
+        btree = builder.allocate_bvh()
+        builder.build(btree)
+        L.LBVH_doCUDASyc()
+        opt = bt.Bittner13GPU(maxNumStepps=args.bittner_steps, numChunks=32, batchPercent=1.0)
+        sah0 = bt.calc_sah(btree, 1.2, 1.0)
+        opt.setup(btree, n)
+        t0 = time.perf_counter()
+        for _ in range(args.bittner_steps):
+            opt.step(btree)
+        L.doCudaDevSync()
+        bt_s = time.perf_counter() - t0
+        sah1 = bt.calc_sah(btree, 1.2, 1.0)
+        bittner = {"steps": args.bittner_steps, "ms_per_step": round(bt_s * 1e3 / args.bittner_steps, 3),
+                   "sah_1.2_1.0_initial": round(sah0, 4), "sah_1.2_1.0_final": round(sah1, 4),
+                   "stats": opt.stats()}
+        opt.free()
+        btree.free()
+        print(f"# bittner: {bittner}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(v, f, bits)
+
+    out = {"metric": METRIC, "value": round(mtri, 2), "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "icosphere clusters + triangle soup, LBVH build + refit, 1920x1080 primary + "
+                                  "1 diffuse bounce", "triangles": n, "spheres": 2048, "soup": 500000,
+                      "morton_bits": bits, "topology": "reference (index Karras)", "seed": 2},
+           "roofline": roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
+           "trace": trace, "bittner": bittner, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(v, f, bits):
+    """Oracle OpenMP LBVH timed on this host (bounded: one build of the same scene)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+
+    threads = orc.lib().orc_omp_max_threads()
+    orc.lbvh_omp(v[:3000], f[:1000] % 3000, bits, threads)  # warm the thread pool
+    reps, secs = 0, 0.0
+    while reps < 3 and secs < 20.0:
+        s, _, _ = orc.lbvh_omp(v, f, bits, threads)
+        secs += s
+        reps += 1
+    mtri = len(f) * reps / secs / 1e6
+    return {"value": round(mtri, 3), "unit": "Mtri/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} full builds of the same {len(f)}-triangle scene ({bits}-bit keys), "
+                      f"oracle OpenMP LBVH, {secs:.2f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--morton-bits", type=int, default=63, choices=[30, 63])
+    ap.add_argument("--bittner-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
+              f"in DESIGN.md and is not run", file=sys.stderr)
+        raise SystemExit(2)
+    run_bench(args)
+
+
+if __name__ == "__main__":
+    main()

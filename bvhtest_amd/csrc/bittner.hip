@@ -1,0 +1,931 @@
+// Bittner et al. 2013 parallel SAH optimizer for gfx950
+// (reference: lib/builder/Bittner13CUDA.cu, lib/base/BVHPatch.hpp).
+//
+// One step (bn13_doAlgorithmStep):
+//   k_cost      removal priority of every inner node, SA^3*2/((L+R)*min(L,R))
+//               (Bittner13CUDA.cu:480-500), as a descending sort key;
+//   onesweep    stable sort of (priority, node id) -- the processing order;
+//   per chunk of K nodes (bn13_rmAndReinsChunk):
+//     k_search  one 64-lane wave per node: remove the node (patch), then two
+//               reinsertions, each found by a wave-parallel exact branch and
+//               bound over the patched view (the reference's findNode1 keeps
+//               a 4096-entry heap per thread in local memory); the patch
+//               locks its nodes with atomicMin(rank);
+//     k_resolve a patch wins iff it holds every lock (lowest rank wins:
+//               deterministic, unlike the reference's first-come CAS);
+//     k_contain a winner whose insertion point lies in a subtree another
+//               winner moves is dropped (two such patches would form a cycle;
+//               the reference has no guard);
+//     k_apply   winners write their topology, all locks are released;
+//     k_refit   two-phase targeted refit from the 3 changed nodes of every
+//               winner (kFixTree3_1 / kFixTree3_2 protocol, agent-scope
+//               acquire/release atomics).  Boxes are exact min/max, so this
+//               equals the reference's full refit (kFixTree1).
+// The root never moves (reinsertion at the root is refused, :310).
+#include "radix_sort.hpp"
+#include "refit.hpp"
+#include "wave.hpp"
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace bvht {
+
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int kNNode = 10;  // BVHPatch.hpp:40
+constexpr uint32_t P_GRAND_PARENT = 0, P_1ST_ROOT = 1, P_1ST_BEST = 2, P_NODE = 3, P_1ST_INSERT = 4, P_2ND_ROOT = 5,
+                   P_2ND_BEST = 6, P_2ND_INSERT = 7, P_SIBLING = 8, P_PARENT = 9;  // BVHPatch.hpp:44-53
+constexpr int kStackCap = 1024;  // per-search frontier (findNode1 heap: 4096)
+constexpr int kStackCapAlt = 16; // altFindNode: bounded frontier (findNode2 keeps 16 slots)
+
+// Stats words (device)
+enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_WORDS = 8 };
+
+struct PatchRec {
+  uint32_t node[kNNode];
+  uint32_t parent[kNNode];
+  uint32_t left[kNNode];
+  uint32_t right[kNNode];
+  uint32_t isLeft[kNNode];
+  uint32_t moved[2];  // reinserted subtree roots
+  uint32_t best[2];   // insertion points
+  uint32_t fix[3];    // refit starts: grandparent, unused1, unused2
+  uint32_t ok;
+};
+
+// BVHPatch (ENABLE_AABB_PATH == 0) in LDS, manipulated by lane 0.
+struct Patch {
+  uint32_t vPatch[kNNode];
+  uint32_t parent[kNNode], left[kNNode], right[kNNode];
+  uint16_t isLeaf[kNNode], isLeft[kNNode];
+  uint32_t path[4];
+  float box[4][6];
+};
+
+struct NodeView {
+  uint32_t parent, left, right, isLeaf, isLeft;
+};
+
+__device__ __forceinline__ NodeView globalView(const bvht_BVHNode *nodes, uint32_t n) {
+  const uint4 w = reinterpret_cast<const uint4 *>(nodes + n)[1];  // maxY maxZ parent left
+  const uint4 z = reinterpret_cast<const uint4 *>(nodes + n)[2];  // right flags level sa
+  return NodeView{w.z, w.w, z.x, z.y & 0xFFFFu, z.y >> 16};
+}
+
+__device__ __forceinline__ void globalBox(const bvht_BVHNode *nodes, uint32_t n, float b[6]) {
+  const uint4 a = reinterpret_cast<const uint4 *>(nodes + n)[0];
+  const uint4 w = reinterpret_cast<const uint4 *>(nodes + n)[1];
+  b[0] = __uint_as_float(a.x);
+  b[1] = __uint_as_float(a.y);
+  b[2] = __uint_as_float(a.z);
+  b[3] = __uint_as_float(a.w);
+  b[4] = __uint_as_float(w.x);
+  b[5] = __uint_as_float(w.y);
+}
+
+__device__ __forceinline__ float sa6(const float b[6]) { return surfaceArea(b[0], b[1], b[2], b[3], b[4], b[5]); }
+__device__ __forceinline__ void merge6(float a[6], const float b[6]) {
+  a[0] = mergeMin(a[0], b[0]);
+  a[1] = mergeMin(a[1], b[1]);
+  a[2] = mergeMin(a[2], b[2]);
+  a[3] = mergeMax(a[3], b[3]);
+  a[4] = mergeMax(a[4], b[4]);
+  a[5] = mergeMax(a[5], b[5]);
+}
+
+// ---- patch operations (lane 0) -------------------------------------------
+
+__device__ void patchClear(Patch &P) {
+  for (int i = 0; i < kNNode; ++i) P.vPatch[i] = NONE;
+  for (int i = 0; i < 4; ++i) P.path[i] = NONE;
+}
+__device__ uint32_t patchIndex(const Patch &P, uint32_t n) {
+  for (int i = 0; i < kNNode; ++i)
+    if (P.vPatch[i] == n) return i;
+  return NONE;
+}
+// patchIndexSubset -- BVHPatch.hpp:134-142
+__device__ __forceinline__ uint32_t patchIndexSubset(const Patch &P, uint32_t n) {
+  if (P.vPatch[0] == n) return 0;
+  if (P.vPatch[1] == NONE) return NONE;
+  if (P.vPatch[1] == n) return 1;
+  if (P.vPatch[2] == n) return 2;
+  if (P.vPatch[3] == n) return 3;
+  return NONE;
+}
+__device__ NodeView patchGet(const Patch &P, const bvht_BVHNode *nodes, uint32_t n) {
+  uint32_t i = patchIndex(P, n);
+  if (i == NONE) return globalView(nodes, n);
+  return NodeView{P.parent[i], P.left[i], P.right[i], P.isLeaf[i], P.isLeft[i]};
+}
+__device__ __forceinline__ NodeView patchGetSubset(const Patch &P, const bvht_BVHNode *nodes, uint32_t n) {
+  uint32_t i = patchIndexSubset(P, n);
+  if (i == NONE) return globalView(nodes, n);
+  return NodeView{P.parent[i], P.left[i], P.right[i], P.isLeaf[i], P.isLeft[i]};
+}
+__device__ void patchNode(Patch &P, const bvht_BVHNode *nodes, uint32_t n, uint32_t slot) {
+  NodeView v = globalView(nodes, n);
+  P.vPatch[slot] = n;
+  P.parent[slot] = v.parent;
+  P.left[slot] = v.left;
+  P.right[slot] = v.right;
+  P.isLeaf[slot] = (uint16_t)v.isLeaf;
+  P.isLeft[slot] = (uint16_t)v.isLeft;
+}
+__device__ void movePatch(Patch &P, uint32_t from, uint32_t to) {
+  P.vPatch[to] = P.vPatch[from];
+  P.vPatch[from] = NONE;
+  P.parent[to] = P.parent[from];
+  P.left[to] = P.left[from];
+  P.right[to] = P.right[from];
+  P.isLeaf[to] = P.isLeaf[from];
+  P.isLeft[to] = P.isLeft[from];
+}
+// getAABB -- BVHPatch.hpp:319-325 (first cached path box that matches)
+__device__ __forceinline__ void patchBox(const Patch &P, const bvht_BVHNode *nodes, uint32_t n, float b[6]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (P.path[i] == n) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) b[k] = P.box[i][k];
+      return;
+    }
+  globalBox(nodes, n, b);
+}
+// patchAABBFrom -- BVHPatch.hpp:293-316
+__device__ void patchAABBFrom(Patch &P, const bvht_BVHNode *nodes, uint32_t node, uint32_t pIdx, uint32_t root) {
+  uint32_t idx = node;
+  NodeView cur = patchGetSubset(P, nodes, idx);
+  float box[6], s[6];
+  patchBox(P, nodes, cur.left, box);
+  bool lastWasLeft = true;
+  for (uint32_t i = 0; i < 2; ++i) {
+    uint32_t sib = lastWasLeft ? cur.right : cur.left;
+    patchBox(P, nodes, sib, s);
+    merge6(box, s);
+    for (int k = 0; k < 6; ++k) P.box[pIdx * 2 + i][k] = box[k];
+    P.path[pIdx * 2 + i] = idx;
+    lastWasLeft = cur.isLeft != 0;
+    if (idx == root) break;
+    idx = cur.parent;
+    cur = patchGetSubset(P, nodes, idx);
+  }
+}
+
+// removeNode -- Bittner13CUDA.cu:267-305 (a patch that cannot be built is
+// dropped whole instead of being applied half done).
+__device__ bool removeNode(Patch &P, const bvht_BVHNode *nodes, uint32_t node, uint32_t root, uint32_t ins[2],
+                           uint32_t unused[2], uint32_t &gpOut) {
+  NodeView nv = globalView(nodes, node);
+  if (nv.isLeaf || node == root) return false;
+  if (nv.parent == root) return false;
+  patchNode(P, nodes, node, P_NODE);
+  NodeView pn = patchGet(P, nodes, P.parent[P_NODE]);
+  uint32_t sib = P.isLeft[P_NODE] == 0 ? pn.left : pn.right;
+  uint32_t parentIdx = P.parent[P_NODE];
+  patchNode(P, nodes, sib, P_SIBLING);
+  patchNode(P, nodes, parentIdx, P_PARENT);
+  uint32_t gp = P.parent[P_PARENT];
+  patchNode(P, nodes, gp, P_GRAND_PARENT);
+  if (P.isLeft[P_PARENT] != 0) {
+    P.left[P_GRAND_PARENT] = sib;
+    P.isLeft[P_SIBLING] = 1;
+  } else {
+    P.right[P_GRAND_PARENT] = sib;
+    P.isLeft[P_SIBLING] = 0;
+  }
+  P.parent[P_SIBLING] = gp;
+  patchAABBFrom(P, nodes, gp, 0, root);
+  const float lsa = nodes[P.left[P_NODE]].surfaceArea, rsa = nodes[P.right[P_NODE]].surfaceArea;
+  if (lsa > rsa) {
+    ins[0] = P.left[P_NODE];
+    ins[1] = P.right[P_NODE];
+  } else {
+    ins[0] = P.right[P_NODE];
+    ins[1] = P.left[P_NODE];
+  }
+  unused[0] = node;
+  unused[1] = parentIdx;
+  gpOut = gp;
+  return true;
+}
+
+// Splice of reinsert -- Bittner13CUDA.cu:312-355 (the search is done before).
+__device__ void spliceAt(Patch &P, const bvht_BVHNode *nodes, uint32_t node, uint32_t unusedIdx, uint32_t bestIdx,
+                         bool update, uint32_t root) {
+  uint32_t bp = patchIndex(P, bestIdx);
+  uint32_t bs;
+  if (bp == NONE) {
+    bs = update ? P_1ST_BEST : P_2ND_BEST;
+    patchNode(P, nodes, bestIdx, bs);
+  } else if (bp == P_GRAND_PARENT) {
+    bs = P_GRAND_PARENT;
+  } else {
+    bs = update ? P_1ST_BEST : P_2ND_BEST;
+    movePatch(P, bp, bs);
+  }
+  const uint32_t ns = update ? P_1ST_INSERT : P_2ND_INSERT;
+  patchNode(P, nodes, node, ns);
+  const uint32_t us = update ? P_NODE : P_PARENT;
+  const uint32_t rootIdx = P.parent[bs];
+  uint32_t rp = patchIndex(P, rootIdx);
+  uint32_t rs = update ? P_1ST_ROOT : P_2ND_ROOT;
+  if (rp == NONE)
+    patchNode(P, nodes, rootIdx, rs);
+  else
+    movePatch(P, rp, rs);
+  // a move may have relocated the best node's slot
+  if (P.vPatch[bs] != bestIdx) bs = patchIndex(P, bestIdx);
+  if (P.isLeft[bs] != 0) {
+    P.left[rs] = unusedIdx;
+    P.isLeft[us] = 1;
+  } else {
+    P.right[rs] = unusedIdx;
+    P.isLeft[us] = 0;
+  }
+  P.parent[us] = rootIdx;
+  P.left[us] = bestIdx;
+  P.right[us] = node;
+  P.parent[bs] = unusedIdx;
+  P.isLeft[bs] = 1;
+  P.parent[ns] = unusedIdx;
+  P.isLeft[ns] = 0;
+  if (update) patchAABBFrom(P, nodes, unusedIdx, 1, root);
+}
+
+// ---- wave-parallel branch and bound (findNode1, Bittner13CUDA.cu:141-191) --
+//
+// Exact: an entry is expanded only while induced + SA(x) < best, which is a
+// lower bound of the cost of every node below it, so the returned node is a
+// cost minimiser of the patched view (ties: smallest total, then smallest
+// node id).  Frontier: LDS stack, up to 64 entries evaluated per step.
+__device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ nodes, uint32_t x, uint32_t root,
+                               uint32_t *s_node, float *s_cost, uint32_t *s_top, int cap, uint32_t *overflow) {
+  const uint32_t lane = threadIdx.x;
+  float xb[6];
+  globalBox(nodes, x, xb);  // getOrig(_n)->bbox
+  const float sa = nodes[x].surfaceArea;
+  if (lane == 0) {
+    s_node[0] = root;
+    s_cost[0] = 0.0f;
+    *s_top = 1;
+  }
+  __syncthreads();
+  float best = HUGE_VALF;
+  uint32_t bestNode = 0;
+  const uint64_t lt = __lanemask_lt();
+  uint32_t top = 1;
+  while (top > 0) {
+    const uint32_t m = top < 64 ? top : 64;
+    const bool active = lane < m;
+    uint32_t node = 0;
+    float ind = 0.0f;
+    if (active) {
+      node = s_node[top - 1 - lane];
+      ind = s_cost[top - 1 - lane];
+    }
+    __syncthreads();
+    const uint32_t newTop = top - m;
+    bool eval = active && (ind + sa) < best;
+    float total = HUGE_VALF, newInd = 0.0f;
+    NodeView nv = {0, 0, 0, 1, 0};
+    if (eval) {
+      nv = patchGetSubset(P, nodes, node);
+      float b[6];
+      patchBox(P, nodes, node, b);
+      newInd = -1.0f * sa6(b);
+      merge6(b, xb);
+      total = ind + sa6(b);
+      newInd += total;
+    }
+    const float mt = waveMin(total);
+    if (mt < best) {
+      uint64_t tied = __ballot(eval && total == mt);
+      uint32_t cand = (eval && total == mt) ? node : NONE;
+      // smallest node id among the tied lanes
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        uint32_t y = __shfl_xor(cand, o, 64);
+        cand = y < cand ? y : cand;
+      }
+      (void)tied;
+      best = mt;
+      bestNode = cand;
+    }
+    const bool push = eval && nv.isLeaf == 0 && (newInd + sa) < best;
+    const uint64_t pm = __ballot(push);
+    const uint32_t pos = newTop + 2u * (uint32_t)__popcll(pm & lt);
+    const uint32_t cnt = 2u * (uint32_t)__popcll(pm);
+    if (push) {
+      if (pos + 1 < (uint32_t)cap) {
+        s_node[pos] = nv.left;
+        s_cost[pos] = newInd;
+        s_node[pos + 1] = nv.right;
+        s_cost[pos + 1] = newInd;
+      }
+    }
+    uint32_t nt = newTop + cnt;
+    if (nt > (uint32_t)cap) {
+      if (lane == 0) atomicAdd(overflow, 1u);
+      nt = (uint32_t)cap & ~1u;
+    }
+    top = nt;
+    __syncthreads();
+  }
+  return bestNode;
+}
+
+__global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ nodes, uint32_t root,
+                                               const uint32_t *__restrict__ sortedIds, uint32_t chunk,
+                                               uint32_t numChunks, uint32_t K, uint32_t offsetAccess, int cap,
+                                               PatchRec *__restrict__ recs, uint32_t *__restrict__ locks,
+                                               uint32_t *__restrict__ stats) {
+  __shared__ Patch P;
+  __shared__ uint32_t s_node[kStackCap];
+  __shared__ float s_cost[kStackCap];
+  __shared__ uint32_t s_top;
+  __shared__ uint32_t s_ok, s_ins[2], s_unused[2], s_gp, s_best[2];
+  const uint32_t k = blockIdx.x, lane = threadIdx.x;
+  PatchRec *rec = recs + k;
+  const uint32_t nodeIdx = sortedIds[offsetAccess ? k * numChunks + chunk : chunk * K + k];
+  if (lane == 0) {
+    patchClear(P);
+    s_ok = removeNode(P, nodes, nodeIdx, root, s_ins, s_unused, s_gp) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_ok) {
+    const uint32_t b1 = waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, cap, stats + ST_OVERFLOW);
+    if (lane == 0) {
+      if (b1 == root) {
+        s_ok = 0;
+      } else {
+        s_best[0] = b1;
+        spliceAt(P, nodes, s_ins[0], s_unused[0], b1, true, root);
+      }
+    }
+    __syncthreads();
+  }
+  if (s_ok) {
+    const uint32_t b2 = waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, cap, stats + ST_OVERFLOW);
+    if (lane == 0) {
+      if (b2 == root) {
+        s_ok = 0;
+      } else {
+        s_best[1] = b2;
+        spliceAt(P, nodes, s_ins[1], s_unused[1], b2, false, root);
+      }
+    }
+    __syncthreads();
+  }
+  if (!s_ok) {
+    if (lane == 0) {
+      rec->ok = 0;
+      atomicAdd(stats + ST_FAILED, 1u);
+    }
+    return;
+  }
+  if (lane < (uint32_t)kNNode) {
+    const uint32_t n = P.vPatch[lane];
+    rec->node[lane] = n;
+    rec->parent[lane] = P.parent[lane];
+    rec->left[lane] = P.left[lane];
+    rec->right[lane] = P.right[lane];
+    rec->isLeft[lane] = P.isLeft[lane];
+    if (n != NONE) atomicMin(locks + n, k);
+  }
+  if (lane == 0) {
+    rec->moved[0] = s_ins[0];
+    rec->moved[1] = s_ins[1];
+    rec->best[0] = s_best[0];
+    rec->best[1] = s_best[1];
+    rec->fix[0] = s_gp;
+    rec->fix[1] = s_unused[0];
+    rec->fix[2] = s_unused[1];
+    rec->ok = 1;
+  }
+}
+
+// A patch wins iff it holds the lock (lowest rank) on every node it touches.
+__global__ __launch_bounds__(256) void k_resolve(PatchRec *__restrict__ recs, uint32_t K,
+                                                 const uint32_t *__restrict__ locks, uint32_t *__restrict__ marks,
+                                                 uint32_t *__restrict__ skipped, uint32_t *__restrict__ stats) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= K) return;
+  PatchRec *r = recs + k;
+  if (!r->ok) return;
+  bool win = true;
+  for (int i = 0; i < kNNode; ++i) {
+    const uint32_t n = r->node[i];
+    if (n != NONE && locks[n] != k) win = false;
+  }
+  if (!win) {
+    r->ok = 2;  // lost a lock
+    skipped[k] += 1;
+    atomicAdd(stats + ST_CONFLICT, 1u);
+    return;
+  }
+  atomicMin(marks + r->moved[0], k);
+  atomicMin(marks + r->moved[1], k);
+}
+
+// Drops a winner whose insertion point lies below a node another winner moves.
+__global__ __launch_bounds__(256) void k_contain(PatchRec *__restrict__ recs, uint32_t K,
+                                                 const bvht_BVHNode *__restrict__ nodes, uint32_t root, uint32_t N,
+                                                 const uint32_t *__restrict__ marks, uint32_t *__restrict__ skipped,
+                                                 uint32_t *__restrict__ stats) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= K) return;
+  PatchRec *r = recs + k;
+  if (r->ok != 1) return;
+  bool bad = false;
+  for (int w = 0; w < 2 && !bad; ++w) {
+    uint32_t x = r->best[w];
+    for (uint32_t steps = 0; steps < N; ++steps) {
+      const uint32_t m = marks[x];
+      if (m != NONE && m != k) {
+        bad = true;
+        break;
+      }
+      if (x == root) break;
+      x = nodes[x].parent;
+    }
+  }
+  if (bad) {
+    r->ok = 3;
+    skipped[k] += 1;
+    atomicAdd(stats + ST_CONTAIN, 1u);
+  }
+}
+
+// Winners write their topology; every patch releases its locks and marks.
+__global__ __launch_bounds__(256) void k_apply(PatchRec *__restrict__ recs, uint32_t K, bvht_BVHNode *__restrict__ nodes,
+                                               uint32_t *__restrict__ locks, uint32_t *__restrict__ marks,
+                                               uint32_t *__restrict__ fixStarts, uint32_t *__restrict__ stats) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= K) return;
+  PatchRec *r = recs + k;
+  const uint32_t ok = r->ok;
+  const bool win = ok == 1;
+  for (int i = 0; i < 3; ++i) fixStarts[3 * k + i] = win ? r->fix[i] : NONE;
+  if (ok == 0) return;
+  for (int i = 0; i < kNNode; ++i) {
+    const uint32_t n = r->node[i];
+    if (n == NONE) continue;
+    locks[n] = NONE;
+    if (win) {  // MiniPatch::applyOne -- BVHPatch.hpp:72-80
+      nodes[n].parent = r->parent[i];
+      nodes[n].left = r->left[i];
+      nodes[n].right = r->right[i];
+      nodes[n].isLeft = (uint16_t)r->isLeft[i];
+    }
+  }
+  marks[r->moved[0]] = NONE;
+  marks[r->moved[1]] = NONE;
+  r->ok = 0;
+  if (win) atomicAdd(stats + ST_APPLIED, 1u);
+}
+
+// kFixTree3_1 -- Bittner13CUDA.cu:407-423: count the arrivals per node.
+__global__ __launch_bounds__(256) void k_refitCount(const uint32_t *__restrict__ starts, uint32_t num,
+                                                    const bvht_BVHNode *__restrict__ nodes, uint32_t N,
+                                                    uint32_t *__restrict__ cnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= num) return;
+  uint32_t node = starts[i];
+  if (node == NONE) return;
+  for (uint32_t steps = 0; steps < N; ++steps) {
+    if (atomicAdd(cnt + node, 1u) != 0) return;
+    const uint32_t p = nodes[node].parent;
+    if (p == node) return;
+    node = p;
+  }
+}
+
+// kFixTree3_2 -- Bittner13CUDA.cu:425-467: the last arrival recomputes.
+__global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__ starts, uint32_t num,
+                                                    bvht_BVHNode *__restrict__ nodes, uint32_t N,
+                                                    uint32_t *__restrict__ cnt) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= num) return;
+  uint32_t node = starts[i];
+  if (node == NONE) return;
+  for (uint32_t steps = 0; steps < N; ++steps) {
+    if (__hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) != 1) return;
+    const uint32_t l = nodes[node].left, r = nodes[node].right;
+    bvht_AABB b = nodes[l].bbox;
+    mergeInto(b, nodes[r].bbox);
+    nodes[node].bbox = b;
+    nodes[node].surfaceArea = surfaceArea(b);
+    const uint32_t p = nodes[node].parent;
+    if (p == node) return;
+    node = p;
+  }
+}
+
+// kCalcCost -- Bittner13CUDA.cu:480-500, written as an ascending sort key
+// (~bits of the non-negative cost: descending cost, ties by input order).
+__global__ __launch_bounds__(256) void k_cost(const bvht_BVHNode *__restrict__ nodes, const uint32_t *__restrict__ ids,
+                                              uint32_t num, uint32_t *__restrict__ keys, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_hist[4][kSortRadix];
+  for (int p = 0; p < 4; ++p) s_hist[p][threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < num; i += gridDim.x * 256) {
+    const bvht_BVHNode nd = nodes[ids[i]];
+    float cost = 0.0f;
+    if (nd.isLeafFlag == 0) {
+      const float s = nd.surfaceArea, l = nodes[nd.left].surfaceArea, r = nodes[nd.right].surfaceArea;
+      cost = (s * s * s * 2.0f) / ((l + r) * (l < r ? l : r));
+    }
+    const uint32_t key = ~__float_as_uint(cost);
+    keys[i] = key;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) atomicAdd(&s_hist[p][(key >> (8 * p)) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < 4; ++p) {
+    const uint32_t c = s_hist[p][threadIdx.x];
+    if (c) atomicAdd(hist + p * kSortRadix + threadIdx.x, c);
+  }
+}
+
+// Order-preserving compaction of node ids by leaf flag (initData's two
+// DeviceSelect::If calls, Bittner13CUDA.cu:962-978): per-block counts, a
+// host-side scan of the (few) block counts, then an ordered write.
+constexpr uint32_t kSelBlock = 256, kSelItems = 16, kSelTile = kSelBlock * kSelItems;
+
+__global__ __launch_bounds__(kSelBlock) void k_selCount(const bvht_BVHNode *__restrict__ nodes, uint32_t N,
+                                                        uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_scan[kSelBlock / 64];
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kSelTile + threadIdx.x * kSelItems;
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < N && nodes[base + j].isLeafFlag == 0) c++;
+  const uint32_t ex = blockExclusiveScan<kSelBlock>(c, s_scan);
+  if (threadIdx.x == kSelBlock - 1) counts[blockIdx.x] = ex + c;
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_selWrite(const bvht_BVHNode *__restrict__ nodes, uint32_t N,
+                                                        const uint32_t *__restrict__ innerOffsets,
+                                                        uint32_t *__restrict__ inner, uint32_t *__restrict__ leaves) {
+  __shared__ uint32_t s_scan[kSelBlock / 64];
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kSelTile + threadIdx.x * kSelItems;
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < N && nodes[base + j].isLeafFlag == 0) c++;
+  uint32_t pi = innerOffsets[blockIdx.x] + blockExclusiveScan<kSelBlock>(c, s_scan);
+  uint32_t pl = base - pi;  // leaves before this thread's first item
+  for (uint32_t j = 0; j < kSelItems; ++j) {
+    const uint32_t id = base + j;
+    if (id >= N) break;
+    if (nodes[id].isLeafFlag == 0)
+      inner[pi++] = id;
+    else
+      leaves[pl++] = id;
+  }
+}
+
+// Deterministic SA sums: one f64 partial per block, summed on the host.
+__global__ __launch_bounds__(256) void k_sahPartials(const bvht_BVHNode *__restrict__ nodes, uint32_t N, uint32_t root,
+                                                     float cInner, float cLeaf, int weighted,
+                                                     double *__restrict__ partials) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+    const bvht_BVHNode &nd = nodes[i];
+    if (weighted) {
+      if (i == root) continue;
+      float c = nd.surfaceArea;
+      c *= nd.isLeafFlag != 0 ? cLeaf : cInner;
+      acc += c;
+    } else {
+      acc += nd.surfaceArea;
+    }
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < (uint32_t)o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+struct Internal {
+  uint32_t N = 0, numLeaves = 0, numInner = 0, K = 0;
+  uint32_t root = 0;
+  uint32_t *inner = nullptr, *leaves = nullptr;
+  uint32_t *keys[2] = {nullptr, nullptr};
+  uint32_t *vals[2] = {nullptr, nullptr};
+  uint32_t *sortScratch = nullptr;
+  SortPlan plan;
+  uint32_t *sorted = nullptr;  // -> vals[result]
+  PatchRec *recs = nullptr;
+  uint32_t *locks = nullptr, *marks = nullptr, *cnt = nullptr, *refitCnt = nullptr;
+  uint32_t *fixStarts = nullptr;
+  uint32_t *skipped = nullptr;
+  uint32_t *stats = nullptr;
+  uint32_t *selCounts = nullptr;
+  double *partials = nullptr;
+  bool selected = false;
+};
+
+constexpr uint32_t kSahBlocks = 512;
+
+Internal *inOf(bvht_GPUWorkingMemory *d) { return d ? reinterpret_cast<Internal *>(d->internal) : nullptr; }
+
+bool validBVH(const bvht_CUDAMemoryBVHPointer *b, const char *what) {
+  if (!b || !b->nodes || b->numNodes == 0) {
+    setError("%s: no device BVH", what);
+    return false;
+  }
+  return true;
+}
+
+double sahSum(const bvht_CUDAMemoryBVHPointer *b, double *partials, uint32_t root, float ci, float cl, int weighted) {
+  k_sahPartials<<<kSahBlocks, 256, 0, stream()>>>(b->nodes, b->numNodes, root, ci, cl, weighted, partials);
+  if (!checkLaunch("k_sahPartials")) return -1.0;
+  std::vector<double> h(kSahBlocks);
+  if (!check(hipMemcpyAsync(h.data(), partials, sizeof(double) * kSahBlocks, hipMemcpyDeviceToHost, stream()),
+             "SAH partials") ||
+      !check(hipStreamSynchronize(stream()), "SAH sync"))
+    return -1.0;
+  double s = 0.0;
+  for (double x : h) s += x;
+  return s;
+}
+
+double *sahScratch() {
+  static double *p = nullptr;
+  if (!p) devMalloc(reinterpret_cast<void **>(&p), sizeof(double) * kSahBlocks, "SAH scratch");
+  return p;
+}
+
+float rootSA(const bvht_CUDAMemoryBVHPointer *b, uint32_t root) {
+  float sa = 0.0f;
+  if (check(hipMemcpyAsync(&sa, &b->nodes[root].surfaceArea, sizeof(float), hipMemcpyDeviceToHost, stream()), "root SA"))
+    check(hipStreamSynchronize(stream()), "root SA sync");
+  return sa;
+}
+
+}  // namespace
+
+}  // namespace bvht
+
+using namespace bvht;
+
+extern "C" {
+
+bvht_GPUWorkingMemory allocateMemory(bvht_CUDAMemoryBVHPointer *bvh, uint32_t batchSize, uint32_t numFaces) {
+  bvht_GPUWorkingMemory m;
+  std::memset(&m, 0, sizeof(m));
+  if (!validBVH(bvh, "allocateMemory")) return m;
+  if (bvh->numNodes != 2 * numFaces - 1 || numFaces < 2) {
+    setError("allocateMemory: %u nodes do not form a binary BVH over %u faces", bvh->numNodes, numFaces);
+    return m;
+  }
+  if (batchSize == 0) {
+    setError("allocateMemory: batch size 0");
+    return m;
+  }
+  Internal *in = new (std::nothrow) Internal();
+  if (!in) {
+    setError("allocateMemory: host allocation failed");
+    return m;
+  }
+  m.internal = in;
+  in->N = bvh->numNodes;
+  in->numLeaves = numFaces;
+  in->numInner = bvh->numNodes - numFaces;
+  in->K = batchSize;
+  in->plan = makeSortPlan(in->numInner, 4, 32);
+  const uint32_t selBlocks = divUp(in->N, kSelTile);
+  m.numLeafNodes = in->numLeaves;
+  m.numInnerNodes = in->numInner;
+  m.numPatches = batchSize;
+  m.numSkipped = batchSize;
+  m.numNodesToFix = 3 * batchSize;
+  m.numFlags = in->N;
+  bool ok = devAlloc(&in->inner, in->numInner, "bn13 inner") && devAlloc(&in->leaves, in->numLeaves, "bn13 leaves") &&
+            devAlloc(&in->keys[0], in->numInner, "bn13 keys") && devAlloc(&in->keys[1], in->numInner, "bn13 keys") &&
+            devAlloc(&in->vals[0], in->numInner, "bn13 vals") && devAlloc(&in->vals[1], in->numInner, "bn13 vals") &&
+            devMalloc(reinterpret_cast<void **>(&in->sortScratch), sortScratchBytes(in->plan), "bn13 sort") &&
+            devAlloc(&in->recs, batchSize, "bn13 patches") && devAlloc(&in->locks, in->N, "bn13 locks") &&
+            devAlloc(&in->marks, in->N, "bn13 marks") && devAlloc(&in->cnt, in->N, "bn13 refit counters") &&
+            devAlloc(&in->refitCnt, in->N, "bn13 refit counters") &&
+            devAlloc(&in->fixStarts, 3 * (size_t)batchSize, "bn13 fix starts") &&
+            devAlloc(&in->skipped, batchSize, "bn13 skipped") && devAlloc(&in->stats, ST_WORDS, "bn13 stats") &&
+            devAlloc(&in->selCounts, selBlocks, "bn13 select") && devAlloc(&in->partials, kSahBlocks, "bn13 SAH");
+  if (!ok) {
+    freeMemory(&m);
+    return m;
+  }
+  m.result = true;
+  return m;
+}
+
+void freeMemory(bvht_GPUWorkingMemory *data) {
+  Internal *in = inOf(data);
+  if (in) {
+    devFree(in->inner);
+    devFree(in->leaves);
+    devFree(in->keys[0]);
+    devFree(in->keys[1]);
+    devFree(in->vals[0]);
+    devFree(in->vals[1]);
+    devFree(in->sortScratch);
+    devFree(in->recs);
+    devFree(in->locks);
+    devFree(in->marks);
+    devFree(in->cnt);
+    devFree(in->refitCnt);
+    devFree(in->fixStarts);
+    devFree(in->skipped);
+    devFree(in->stats);
+    devFree(in->selCounts);
+    devFree(in->partials);
+    delete in;
+  }
+  if (data) {
+    std::memset(data, 0, sizeof(*data));
+  }
+}
+
+void initData(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t blockSize) {
+  (void)blockSize;
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "initData")) {
+    if (!in) setError("initData: working memory not allocated");
+    return;
+  }
+  // root from the device header (the GPU optimizer keeps it fixed)
+  bvht_BVH hdr;
+  if (!check(hipMemcpyAsync(&hdr, bvh->bvh, sizeof(hdr), hipMemcpyDeviceToHost, stream()), "initData header") ||
+      !check(hipStreamSynchronize(stream()), "initData sync"))
+    return;
+  in->root = hdr.vRootIndex;
+  const uint32_t selBlocks = divUp(in->N, kSelTile);
+  k_selCount<<<selBlocks, kSelBlock, 0, stream()>>>(bvh->nodes, in->N, in->selCounts);
+  if (!checkLaunch("k_selCount")) return;
+  std::vector<uint32_t> counts(selBlocks);
+  if (!check(hipMemcpyAsync(counts.data(), in->selCounts, 4ull * selBlocks, hipMemcpyDeviceToHost, stream()),
+             "initData counts") ||
+      !check(hipStreamSynchronize(stream()), "initData sync"))
+    return;
+  uint32_t total = 0;
+  for (auto &c : counts) {
+    uint32_t t = c;
+    c = total;
+    total += t;
+  }
+  if (total != in->numInner) {
+    setError("initData: %u inner nodes found, expected %u", total, in->numInner);
+    return;
+  }
+  if (!check(hipMemcpyAsync(in->selCounts, counts.data(), 4ull * selBlocks, hipMemcpyHostToDevice, stream()),
+             "initData offsets"))
+    return;
+  k_selWrite<<<selBlocks, kSelBlock, 0, stream()>>>(bvh->nodes, in->N, in->selCounts, in->inner, in->leaves);
+  if (!checkLaunch("k_selWrite")) return;
+  fillU32(in->locks, NONE, in->N);
+  fillU32(in->marks, NONE, in->N);
+  fillU32(in->cnt, 0u, in->N);
+  fillU32(in->refitCnt, 0u, in->N);
+  fillU32(in->skipped, 0u, in->K);
+  fillU32(in->stats, 0u, ST_WORDS);
+  fillU32(in->fixStarts, NONE, 3ull * in->K);
+  check(hipMemsetAsync(in->recs, 0, sizeof(PatchRec) * in->K, stream()), "initData patches");
+  check(hipStreamSynchronize(stream()), "initData sync");
+}
+
+void fixTree1(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t blockSize) {
+  (void)blockSize;
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "fixTree1")) return;
+  refitFromLeaves(bvh->nodes, 0, in->numLeaves, in->leaves, in->refitCnt, in->N);
+}
+
+void fixTree3(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t blockSize) {
+  (void)blockSize;
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "fixTree3")) return;
+  const uint32_t num = 3 * in->K;
+  k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
+  if (!checkLaunch("k_refitCount")) return;
+  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
+  checkLaunch("k_refitApply");
+}
+
+void bn13_selectNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg) {
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "bn13_selectNodes")) return;
+  if (!cfg.sort) {  // Bittner13CUDA.cu:719
+    setError("bn13_selectNodes: topKThElement is not supported (set option sort)");
+    return;
+  }
+  uint32_t *scratch = in->sortScratch;
+  if (!sortReset(in->plan, scratch)) return;
+  uint32_t blocks = divUp(in->numInner, 256 * 8);
+  if (blocks > (uint32_t)numCUs() * 8) blocks = numCUs() * 8;
+  k_cost<<<blocks, 256, 0, stream()>>>(bvh->nodes, in->inner, in->numInner, in->keys[0], sortHist(in->plan, scratch));
+  if (!checkLaunch("k_cost")) return;
+  void *keys[2] = {in->keys[0], in->keys[1]};
+  uint32_t *vals[2] = {in->vals[0], in->vals[1]};
+  // values: node ids from the inner list (copied into vals[0])
+  if (!check(hipMemcpyAsync(in->vals[0], in->inner, 4ull * in->numInner, hipMemcpyDeviceToDevice, stream()),
+             "bn13 ids"))
+    return;
+  const int r = sortRun(in->plan, keys, vals, false, false, scratch);
+  if (r < 0) return;
+  in->sorted = in->vals[r];
+  in->selected = true;
+}
+
+void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg, uint32_t chunk) {
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "bn13_rmAndReinsChunk")) return;
+  if (!in->selected) {
+    setError("bn13_rmAndReinsChunk: call bn13_selectNodes first");
+    return;
+  }
+  const uint32_t K = cfg.chunkSize;
+  if (K == 0 || K > in->K || (uint64_t)cfg.numChunks * K > in->numInner || chunk >= cfg.numChunks) {
+    setError("bn13_rmAndReinsChunk: chunk %u of %u x %u does not fit %u patches / %u inner nodes", chunk, cfg.numChunks,
+             K, in->K, in->numInner);
+    return;
+  }
+  const int cap = cfg.altFindNode ? kStackCapAlt : kStackCap;
+  k_search<<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->sorted, chunk, cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u,
+                                   cap, in->recs, in->locks, in->stats);
+  if (!checkLaunch("k_search")) return;
+  k_resolve<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats);
+  if (!checkLaunch("k_resolve")) return;
+  k_contain<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, bvh->nodes, in->root, in->N, in->marks, in->skipped,
+                                                 in->stats);
+  if (!checkLaunch("k_contain")) return;
+  k_apply<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, bvh->nodes, in->locks, in->marks, in->fixStarts, in->stats);
+  if (!checkLaunch("k_apply")) return;
+  const uint32_t num = 3 * K;
+  k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
+  if (!checkLaunch("k_refitCount")) return;
+  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
+  checkLaunch("k_refitApply");
+}
+
+void bn13_doAlgorithmStep(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg) {
+  bn13_selectNodes(data, bvh, cfg);
+  for (uint32_t i = 0; i < cfg.numChunks; ++i) bn13_rmAndReinsChunk(data, bvh, cfg, i);
+}
+
+float CUDAcalcSAH(bvht_CUDAMemoryBVHPointer *bvh) {
+  if (!validBVH(bvh, "CUDAcalcSAH")) return 0.0f;
+  double *p = sahScratch();
+  if (!p) return 0.0f;
+  const double s = sahSum(bvh, p, 0, 1.0f, 1.0f, 0);
+  const float r = rootSA(bvh, 0);  // the reference reads node 0 (Bittner13CUDA.cu:794)
+  return (float)((s - (double)r) / (double)r);
+}
+
+double bvht_calcSAHWeighted(bvht_CUDAMemoryBVHPointer *bvh, float cInner, float cLeaf) {
+  if (!validBVH(bvh, "bvht_calcSAHWeighted")) return 0.0;
+  bvht_BVH hdr;
+  if (!check(hipMemcpyAsync(&hdr, bvh->bvh, sizeof(hdr), hipMemcpyDeviceToHost, stream()), "SAH header") ||
+      !check(hipStreamSynchronize(stream()), "SAH sync"))
+    return 0.0;
+  double *p = sahScratch();
+  if (!p) return 0.0;
+  const double s = sahSum(bvh, p, hdr.vRootIndex, cInner, cLeaf, 1);
+  return s / (double)rootSA(bvh, hdr.vRootIndex);
+}
+
+void doCudaDevSync(void) { check(hipStreamSynchronize(stream()), "doCudaDevSync"); }
+
+uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data) {
+  Internal *in = inOf(data);
+  if (!in) return 0;
+  std::vector<uint32_t> h(in->K);
+  if (!check(hipMemcpyAsync(h.data(), in->skipped, 4ull * in->K, hipMemcpyDeviceToHost, stream()), "calcNumSkipped") ||
+      !check(hipStreamSynchronize(stream()), "calcNumSkipped sync"))
+    return 0;
+  uint32_t s = 0;
+  for (uint32_t x : h) s += x;
+  return s;
+}
+
+// Counters of the optimizer (extension): conflicts, failed patches,
+// containment drops, frontier overflows, applied patches.
+int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
+  Internal *in = inOf(data);
+  if (!in || !out) return 0;
+  uint32_t h[ST_WORDS];
+  if (!check(hipMemcpyAsync(h, in->stats, sizeof(h), hipMemcpyDeviceToHost, stream()), "bn13 stats") ||
+      !check(hipStreamSynchronize(stream()), "bn13 stats sync"))
+    return 0;
+  int n = maxWords < 5 ? maxWords : 5;
+  for (int i = 0; i < n; ++i) out[i] = h[i];
+  return n;
+}
+
+}  // extern "C"

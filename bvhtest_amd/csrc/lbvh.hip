@@ -1,0 +1,681 @@
+// LBVH builder for gfx950 (reference: lib/builder/LBVH_CUDA.cu).
+//
+// Pipeline of one build (all on one stream, no host round trip):
+//   k_sceneBox   faces + vertices -> scene box (union of the +-FLT_EPSILON
+//                triangle boxes and the origin), offset, scale; also zeroes
+//                the sort state.  Last-block finalisation, one launch.
+//   k_morton     centroid -> 30-bit (reference) or 63-bit Morton key, plus
+//                the digit histograms of every sort pass (LDS, then one
+//                global atomic per digit per block).
+//   onesweep     stable radix sort of (key, face index); the last pass keeps
+//                only the face indices when the topology does not need keys.
+//   k_buildLevel tiles of 2^kTileLog units: level 0 gathers each sorted
+//                triangle, writes the leaf, then reduces the tile's subtree
+//                in LDS and writes each inner node once (box, topology, SA);
+//                further launches reduce tile roots the same way.
+// The reference topology is a closed form of n (see types.hpp), so no key is
+// read to build it and no global atomic is needed for the refit.
+#include "lbvh.hpp"
+#include "radix_sort.hpp"
+#include "refit.hpp"
+#include "wave.hpp"
+
+#include <cstring>
+#include <new>
+
+namespace bvht {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTileLog = 10;  // 1024 units per build workgroup
+constexpr uint32_t kBoxBlocksMax = 1024;
+
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+// Writes one full 48-byte node with three 16-byte stores.
+__device__ __forceinline__ void storeNode(bvht_BVHNode *dst, const bvht_AABB &b, uint32_t parent, uint32_t left,
+                                          uint32_t right, uint32_t isLeaf, uint32_t isLeft, uint32_t level, float sa) {
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+  d[0] = make_uint4(f2u(b.minMax[0].x), f2u(b.minMax[0].y), f2u(b.minMax[0].z), f2u(b.minMax[1].x));
+  d[1] = make_uint4(f2u(b.minMax[1].y), f2u(b.minMax[1].z), parent, left);
+  d[2] = make_uint4(right, isLeaf | (isLeft << 16), level, f2u(sa));
+}
+
+__device__ __forceinline__ bvht_vec3 ldVec3(const bvht_vec3 *p) { return *p; }
+
+__global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict__ vert,
+                                                     const bvht_Triangle *__restrict__ faces, uint32_t n,
+                                                     BuildParams *__restrict__ prm, float *__restrict__ partials,
+                                                     uint32_t *__restrict__ zeroPtr, size_t zeroWords) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (size_t i = (size_t)blockIdx.x * kBlock + tid; i < zeroWords; i += (size_t)gridDim.x * kBlock) zeroPtr[i] = 0;
+  // the reduction starts from the all-zero box (LBVH_CUDA.cu:104-106)
+  float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+  for (uint32_t i = blockIdx.x * kBlock + tid; i < n; i += gridDim.x * kBlock) {
+    bvht_Triangle t = faces[i];
+    bvht_AABB b = triangleBox(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+    mn[0] = mergeMin(mn[0], b.minMax[0].x);
+    mn[1] = mergeMin(mn[1], b.minMax[0].y);
+    mn[2] = mergeMin(mn[2], b.minMax[0].z);
+    mx[0] = mergeMax(mx[0], b.minMax[1].x);
+    mx[1] = mergeMax(mx[1], b.minMax[1].y);
+    mx[2] = mergeMax(mx[2], b.minMax[1].z);
+  }
+  __shared__ float s_red[6][kBlock / 64];
+  __shared__ bool s_last;
+  for (int a = 0; a < 3; ++a) {
+    float x = waveMin(mn[a]), y = waveMax(mx[a]);
+    if (lane == 0) {
+      s_red[a][wave] = x;
+      s_red[3 + a][wave] = y;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      float x = s_red[a][0], y = s_red[3 + a][0];
+      for (int w = 1; w < kBlock / 64; ++w) {
+        x = mergeMin(x, s_red[a][w]);
+        y = mergeMax(y, s_red[3 + a][w]);
+      }
+      partials[blockIdx.x * 6 + a] = x;
+      partials[blockIdx.x * 6 + 3 + a] = y;
+    }
+    __threadfence();
+    s_last = atomicAdd(&prm->done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = 0.0f;
+    mx[a] = 0.0f;
+  }
+  for (uint32_t b = tid; b < gridDim.x; b += kBlock)
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = mergeMin(mn[a], partials[b * 6 + a]);
+      mx[a] = mergeMax(mx[a], partials[b * 6 + 3 + a]);
+    }
+  __syncthreads();
+  for (int a = 0; a < 3; ++a) {
+    float x = waveMin(mn[a]), y = waveMax(mx[a]);
+    if (lane == 0) {
+      s_red[a][wave] = x;
+      s_red[3 + a][wave] = y;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      float x = s_red[a][0], y = s_red[3 + a][0];
+      for (int w = 1; w < kBlock / 64; ++w) {
+        x = mergeMin(x, s_red[a][w]);
+        y = mergeMax(y, s_red[3 + a][w]);
+      }
+      prm->sceneMin[a] = x;
+      prm->sceneMax[a] = y;
+      // LBVH_calcMortonCodes host part (LBVH_CUDA.cu:171-180), f32, IEEE division
+      float off = 0.0f - x;
+      prm->offset[a] = off;
+      prm->scale[a] = __fdiv_rn(1.0f, y + off);
+    }
+    prm->done = 0;
+  }
+}
+
+template <typename K, int PASSES>
+__global__ __launch_bounds__(kBlock) void k_morton(const bvht_vec3 *__restrict__ vert,
+                                                   const bvht_Triangle *__restrict__ faces, uint32_t n,
+                                                   const BuildParams *__restrict__ prm, K *__restrict__ keys,
+                                                   uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_hist[PASSES][kSortRadix];
+  for (int p = 0; p < PASSES; ++p) s_hist[p][threadIdx.x] = 0;
+  __syncthreads();
+  const bvht_vec3 off = {prm->offset[0], prm->offset[1], prm->offset[2]};
+  const bvht_vec3 sc = {prm->scale[0], prm->scale[1], prm->scale[2]};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    bvht_Triangle t = faces[i];
+    bvht_vec3 c = centroid(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+    K code;
+    if constexpr (sizeof(K) == 8)
+      code = morton63(c, off, sc);
+    else
+      code = morton30(c, off, sc);
+    keys[i] = code;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) atomicAdd(&s_hist[p][(uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1)], 1u);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PASSES; ++p) {
+    uint32_t c = s_hist[p][threadIdx.x];
+    if (c) atomicAdd(&hist[p * kSortRadix + threadIdx.x], c);
+  }
+}
+
+// Parent index and isLeft flag of the node covering [l, r].
+__device__ __forceinline__ void parentOf(uint32_t l, uint32_t r, uint32_t N1, uint32_t &parent, uint32_t &isLeft) {
+  if (l == 0 && r == N1) {
+    parent = 0;  // the root is its own parent
+    isLeft = 0;
+    return;
+  }
+  uint32_t pl, pr;
+  rangeParent(l, r, N1, pl, pr);
+  parent = rangeNodeIndex(pl, pr, N1);
+  isLeft = rangeIsLeft(l, r, N1) ? 1u : 0u;
+}
+
+// One level of the tile hierarchy.  Units are aligned runs of 2^unitLog leaf
+// positions; a workgroup owns 2^kTileLog consecutive units and writes every
+// node whose leaf range is a union of 2..2^kTileLog of its units.
+template <bool LEVEL0>
+__global__ __launch_bounds__(kBlock) void k_buildLevel(bvht_BVHNode *__restrict__ nodes,
+                                                       const uint32_t *__restrict__ sortedFaces,
+                                                       const bvht_Triangle *__restrict__ faces,
+                                                       const bvht_vec3 *__restrict__ vert, uint32_t n,
+                                                       uint32_t unitLog) {
+  constexpr uint32_t T = 1u << kTileLog;
+  __shared__ float s_box[6][T];
+  const uint32_t N1 = n - 1;
+  const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
+  const uint64_t u0 = (uint64_t)blockIdx.x << kTileLog;
+  const uint32_t cnt = (uint32_t)min((uint64_t)T, U - u0);
+
+  for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+    const uint64_t u = u0 + k;
+    bvht_AABB b;
+    if (LEVEL0) {
+      const uint32_t p = (uint32_t)u;
+      uint32_t f = sortedFaces[p];
+      if (f >= n) f = 0;  // never read outside the mesh, even from a corrupt permutation
+      const bvht_Triangle t = faces[f];
+      b = triangleBox(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+      uint32_t parent, isLeft;
+      parentOf(p, p, N1, parent, isLeft);
+      // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
+      storeNode(nodes + N1 + p, b, parent, f, 1u, 1u, isLeft, 255u, surfaceArea(b));
+    } else {
+      const uint32_t l = (uint32_t)(u << unitLog);
+      const uint32_t r = (uint32_t)min((u + 1) << unitLog, (uint64_t)n) - 1;
+      const float *src = reinterpret_cast<const float *>(nodes + rangeNodeIndex(l, r, N1));
+      b.minMax[0] = {src[0], src[1], src[2]};
+      b.minMax[1] = {src[3], src[4], src[5]};
+    }
+    s_box[0][k] = b.minMax[0].x;
+    s_box[1][k] = b.minMax[0].y;
+    s_box[2][k] = b.minMax[0].z;
+    s_box[3][k] = b.minMax[1].x;
+    s_box[4][k] = b.minMax[1].y;
+    s_box[5][k] = b.minMax[1].z;
+  }
+  __syncthreads();
+
+  for (uint32_t j = 1; j <= kTileLog; ++j) {
+    const uint32_t half = 1u << (j - 1);
+    const uint32_t nb = (cnt + (1u << j) - 1) >> j;
+    for (uint32_t a = threadIdx.x; a < nb; a += kBlock) {
+      const uint32_t k0 = a << j, km = k0 + half;
+      if (km >= cnt) continue;  // right half empty: same range as the left half
+      bvht_AABB b;
+      b.minMax[0] = {mergeMin(s_box[0][k0], s_box[0][km]), mergeMin(s_box[1][k0], s_box[1][km]),
+                     mergeMin(s_box[2][k0], s_box[2][km])};
+      b.minMax[1] = {mergeMax(s_box[3][k0], s_box[3][km]), mergeMax(s_box[4][k0], s_box[4][km]),
+                     mergeMax(s_box[5][k0], s_box[5][km])};
+      s_box[0][k0] = b.minMax[0].x;
+      s_box[1][k0] = b.minMax[0].y;
+      s_box[2][k0] = b.minMax[0].z;
+      s_box[3][k0] = b.minMax[1].x;
+      s_box[4][k0] = b.minMax[1].y;
+      s_box[5][k0] = b.minMax[1].z;
+      const uint32_t l = (uint32_t)((u0 + k0) << unitLog);
+      const uint32_t m = (uint32_t)((u0 + km) << unitLog);
+      const uint32_t r = (uint32_t)min((u0 + k0 + (1u << j)) << unitLog, (uint64_t)n) - 1;
+      uint32_t parent, isLeft;
+      parentOf(l, r, N1, parent, isLeft);
+      storeNode(nodes + rangeNodeIndex(l, r, N1), b, parent, rangeNodeIndex(l, m - 1, N1), rangeNodeIndex(m, r, N1), 0u,
+                isLeft, 0u, surfaceArea(b));
+    }
+    __syncthreads();
+  }
+}
+
+// Karras 2012 over the whole key (morton, index) -- extension (topology 1).
+// delta(i, j) = common prefix length of (key_i, i) and (key_j, j).
+template <typename K>
+__device__ __forceinline__ int keyDelta(const K *__restrict__ keys, uint32_t n, uint32_t i, uint32_t j) {
+  if (j >= n) return -1;
+  K a = keys[i], b = keys[j];
+  constexpr int W = sizeof(K) * 8;
+  if (a == b) return W + __clz(i ^ j);
+  if constexpr (sizeof(K) == 8)
+    return __clzll((long long)(a ^ b));
+  else
+    return __clz((int)(a ^ b));
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_karras(bvht_BVHNode *__restrict__ nodes, const K *__restrict__ keys,
+                                                   uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i + 1 >= n) return;
+  int d = keyDelta(keys, n, i, i + 1) - keyDelta(keys, n, i, i - 1);
+  d = d > 0 ? 1 : -1;
+  const int dmin = keyDelta(keys, n, i, i - (uint32_t)d);
+  uint32_t lMax = 2;
+  while (keyDelta(keys, n, i, i + lMax * (uint32_t)d) > dmin) lMax *= 2;
+  uint32_t l = 0;
+  do {
+    lMax >>= 1;
+    if (keyDelta(keys, n, i, i + (l + lMax) * (uint32_t)d) > dmin) l += lMax;
+  } while (lMax > 1);
+  const uint32_t j = i + l * (uint32_t)d;
+  const uint32_t first = min(i, j), last = max(i, j);
+  const int common = keyDelta(keys, n, first, last);
+  uint32_t split = first, step = last - first;
+  do {
+    step = (step + 1) >> 1;
+    const uint32_t ns = split + step;
+    if (ns < last && keyDelta(keys, n, first, ns) > common) split = ns;
+  } while (step > 1);
+  uint32_t a = split, b = split + 1;
+  if (a == first) a += n - 1;
+  if (b == last) b += n - 1;
+  nodes[i].left = a;
+  nodes[i].right = b;
+  nodes[i].isLeafFlag = 0;
+  nodes[i].level = 0;
+  nodes[i]._pad = 0;
+  nodes[a].parent = i;
+  nodes[a].isLeft = 1;
+  nodes[b].parent = i;
+  nodes[b].isLeft = 0;
+  if (i == 0) {
+    nodes[0].parent = 0;
+    nodes[0].isLeft = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaves(bvht_BVHNode *__restrict__ nodes,
+                                                   const uint32_t *__restrict__ sortedFaces,
+                                                   const bvht_Triangle *__restrict__ faces,
+                                                   const bvht_vec3 *__restrict__ vert, uint32_t n) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t f = sortedFaces[p];
+  const bvht_Triangle t = faces[f];
+  bvht_AABB b = triangleBox(vert[t.v1], vert[t.v2], vert[t.v3]);
+  // parent / isLeft are written by k_karras (kGenLeafNodes writes UINT32_MAX)
+  storeNode(nodes + n - 1 + p, b, n == 1 ? 0u : UINT32_MAX, f, 1u, 1u, 0u, 255u, surfaceArea(b));
+}
+
+struct Internal {
+  SortPlan plan;
+  float *partials = nullptr;
+  uint32_t boxBlocks = 0;
+  int resultBuf = 0;
+  bvht_MeshRaw mesh = {};  // mesh of the current build (staged API)
+  bool haveMesh = false;
+  BuildParams hostParams = {};
+};
+
+Internal *internalOf(bvht_LBVH_WorkingMemory *m) { return reinterpret_cast<Internal *>(m->internal); }
+
+struct Timing {
+  static constexpr int kMax = 32;
+  bool enabled = false;
+  hipEvent_t ev[kMax] = {};
+  const char *name[kMax] = {};
+  bool created = false;
+  int recorded = 0;
+  void start() { recorded = 0; }
+  void record(const char *label) {
+    if (!enabled || recorded >= kMax) return;
+    if (!created) {
+      for (auto &e : ev) (void)hipEventCreate(&e);
+      created = true;
+    }
+    (void)hipEventRecord(ev[recorded], stream());
+    name[recorded] = label;
+    recorded++;
+  }
+};
+Timing &timing() {
+  static Timing t;
+  return t;
+}
+
+bool validateMesh(const bvht_MeshRaw *mesh, const char *what) {
+  if (!mesh || mesh->numFaces == 0 || !mesh->faces || !mesh->vert) {
+    setError("%s: empty or missing mesh", what);
+    return false;
+  }
+  if (mesh->numFaces > (1u << 30)) {
+    setError("%s: %u faces exceed the 2^30 limit", what, mesh->numFaces);
+    return false;
+  }
+  return true;
+}
+
+uint32_t boxBlocks(uint32_t n) {
+  uint32_t b = divUp(n, kBlock * 8);
+  uint32_t cap = std::min<uint32_t>(kBoxBlocksMax, (uint32_t)numCUs() * 4);
+  return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
+uint32_t mortonBlocks(uint32_t n) {
+  uint32_t b = divUp(n, kBlock * 4);
+  uint32_t cap = (uint32_t)numCUs() * 8;
+  return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
+bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
+  Internal *in = internalOf(mem);
+  uint32_t n = mesh->numFaces;
+  uint32_t *hist = sortHist(in->plan, (uint32_t *)mem->sortScratch);
+  if (mem->opts.mortonBits == 63)
+    k_morton<uint64_t, 8><<<mortonBlocks(n), kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
+                                                                    (uint64_t *)mem->keys[0], hist);
+  else
+    k_morton<uint32_t, 4><<<mortonBlocks(n), kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
+                                                                    (uint32_t *)mem->keys[0], hist);
+  return checkLaunch("k_morton");
+}
+
+bool runSort(bvht_LBVH_WorkingMemory *mem) {
+  Internal *in = internalOf(mem);
+  const bool keepKeys = mem->opts.topology == 1;
+  Timing &tm = timing();
+  int r = sortRun(in->plan, mem->keys, mem->values, true, keepKeys, (uint32_t *)mem->sortScratch,
+                  tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr);
+  if (r < 0) return false;
+  in->resultBuf = r;
+  mem->sortedValues = mem->values[r];
+  mem->sortedKeys = keepKeys ? mem->keys[r] : nullptr;
+  return true;
+}
+
+bool runTree(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh) {
+  const uint32_t n = mesh->numFaces;
+  if (mem->opts.topology == 1) {
+    k_leaves<<<divUp(n, kBlock), kBlock, 0, stream()>>>(bvh->nodes, mem->sortedValues, mesh->faces, mesh->vert, n);
+    if (!checkLaunch("k_leaves")) return false;
+    if (n > 1) {
+      if (mem->opts.mortonBits == 63)
+        k_karras<uint64_t><<<divUp(n, kBlock), kBlock, 0, stream()>>>(bvh->nodes, (const uint64_t *)mem->sortedKeys, n);
+      else
+        k_karras<uint32_t><<<divUp(n, kBlock), kBlock, 0, stream()>>>(bvh->nodes, (const uint32_t *)mem->sortedKeys, n);
+      if (!checkLaunch("k_karras")) return false;
+      return refitFromLeaves(bvh->nodes, n - 1, n, nullptr, mem->atomicLocks, mem->numLocks);
+    }
+    return true;
+  }
+  uint32_t unitLog = 0;
+  bool first = true;
+  while (true) {
+    const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
+    const uint32_t blocks = (uint32_t)((U + (1u << kTileLog) - 1) >> kTileLog);
+    if (first)
+      k_buildLevel<true><<<blocks, kBlock, 0, stream()>>>(bvh->nodes, mem->sortedValues, mesh->faces, mesh->vert, n, 0);
+    else
+      k_buildLevel<false><<<blocks, kBlock, 0, stream()>>>(bvh->nodes, nullptr, mesh->faces, mesh->vert, n, unitLog);
+    if (!checkLaunch("k_buildLevel")) return false;
+    timing().record(first ? "k_buildLevel0" : "k_buildLevelUp");
+    first = false;
+    if (blocks <= 1) break;
+    unitLog += kTileLog;
+  }
+  return true;
+}
+
+}  // namespace
+
+}  // namespace bvht
+
+using namespace bvht;
+
+extern "C" {
+
+bvht_LBVH_Options bvht_LBVH_defaultOptions(void) {
+  bvht_LBVH_Options o;
+  o.mortonBits = 30;
+  o.topology = 0;
+  o.tileLog2 = kTileLog;
+  o.digitBits = kSortRadixBits;
+  return o;
+}
+
+bool LBVH_allocateBVH(bvht_CUDAMemoryBVHPointer *bvh, const bvht_MeshRaw *mesh) {
+  if (!bvh || !validateMesh(mesh, "LBVH_allocateBVH")) return false;
+  if (bvh->bvh != nullptr || bvh->nodes != nullptr) {  // LBVH_CUDA.cu:400
+    setError("LBVH_allocateBVH: BVH memory already allocated");
+    return false;
+  }
+  bvh->numNodes = mesh->numFaces * 2 - 1;
+  bvht_BVH hdr;
+  std::memset(&hdr, 0, sizeof(hdr));
+  if (!devAlloc(&bvh->bvh, 1, "LBVH_allocateBVH header") || !devAlloc(&bvh->nodes, bvh->numNodes, "LBVH_allocateBVH nodes"))
+    goto error;
+  hdr.bvh = bvh->nodes;
+  hdr.vSize = bvh->numNodes;
+  hdr.vCapacity = bvh->numNodes;
+  hdr.vRootIndex = 0;
+  if (!check(hipMemcpyAsync(bvh->bvh, &hdr, sizeof(hdr), hipMemcpyHostToDevice, stream()), "LBVH_allocateBVH header") ||
+      !check(hipStreamSynchronize(stream()), "LBVH_allocateBVH sync"))
+    goto error;
+  return true;
+error:
+  devFree(bvh->nodes);
+  devFree(bvh->bvh);
+  bvh->nodes = nullptr;
+  bvh->bvh = nullptr;
+  bvh->numNodes = 0;
+  return false;
+}
+
+bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *mesh, const bvht_LBVH_Options *opts) {
+  bvht_LBVH_WorkingMemory m;
+  std::memset(&m, 0, sizeof(m));
+  m.opts = opts ? *opts : bvht_LBVH_defaultOptions();
+  if (m.opts.mortonBits != 30 && m.opts.mortonBits != 63) {
+    setError("LBVH_allocateWorkingMemory: mortonBits must be 30 or 63 (got %u)", m.opts.mortonBits);
+    return m;
+  }
+  if (m.opts.topology > 1) {
+    setError("LBVH_allocateWorkingMemory: topology must be 0 or 1 (got %u)", m.opts.topology);
+    return m;
+  }
+  m.opts.tileLog2 = kTileLog;
+  m.opts.digitBits = kSortRadixBits;
+  if (!validateMesh(mesh, "LBVH_allocateWorkingMemory")) return m;
+  const uint32_t n = mesh->numFaces;
+  const int keyBytes = m.opts.mortonBits == 63 ? 8 : 4;
+  Internal *in = new (std::nothrow) Internal();
+  if (!in) {
+    setError("LBVH_allocateWorkingMemory: host allocation failed");
+    return m;
+  }
+  m.internal = in;
+  m.numFaces = n;
+  in->plan = makeSortPlan(n, keyBytes, (int)m.opts.mortonBits);
+  in->boxBlocks = boxBlocks(n);
+  m.sortScratchBytes = sortScratchBytes(in->plan);
+  m.numLocks = 2 * n - 1;
+  if (!devMalloc(&m.keys[0], (size_t)keyBytes * n, "LBVH keys") || !devMalloc(&m.keys[1], (size_t)keyBytes * n, "LBVH keys") ||
+      !devAlloc(&m.values[0], n, "LBVH values") || !devAlloc(&m.values[1], n, "LBVH values") ||
+      !devMalloc(&m.sortScratch, m.sortScratchBytes, "LBVH sort scratch") ||
+      !devMalloc(&m.params, sizeof(BuildParams), "LBVH params") ||
+      !devAlloc(&in->partials, (size_t)kBoxBlocksMax * 6, "LBVH partials") ||
+      !devAlloc(&m.atomicLocks, m.numLocks, "LBVH locks")) {
+    LBVH_freeWorkingMemory(&m);
+    return m;
+  }
+  if (!check(hipMemsetAsync(m.params, 0, sizeof(BuildParams), stream()), "LBVH params init") ||
+      !check(hipMemsetAsync(m.atomicLocks, 0, sizeof(uint32_t) * m.numLocks, stream()), "LBVH locks init") ||
+      !check(hipStreamSynchronize(stream()), "LBVH init sync")) {
+    LBVH_freeWorkingMemory(&m);
+    return m;
+  }
+  m.lRes = true;
+  return m;
+}
+
+bvht_LBVH_WorkingMemory LBVH_allocateWorkingMemory(const bvht_MeshRaw *mesh) {
+  return bvht_LBVH_allocateWorkingMemoryEx(mesh, nullptr);
+}
+
+void LBVH_freeWorkingMemory(bvht_LBVH_WorkingMemory *mem) {
+  if (!mem) return;
+  Internal *in = internalOf(mem);
+  if (in) devFree(in->partials);
+  delete in;
+  devFree(mem->keys[0]);
+  devFree(mem->keys[1]);
+  devFree(mem->values[0]);
+  devFree(mem->values[1]);
+  devFree(mem->sortScratch);
+  devFree(mem->params);
+  devFree(mem->atomicLocks);
+  bvht_LBVH_Options o = mem->opts;
+  std::memset(mem, 0, sizeof(*mem));
+  mem->opts = o;
+}
+
+static bool checkMem(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, const char *what) {
+  if (!mem || !mem->lRes || !mem->internal) {
+    setError("%s: working memory not allocated", what);
+    return false;
+  }
+  if (!validateMesh(mesh, what)) return false;
+  if (mesh->numFaces != mem->numFaces) {
+    setError("%s: mesh has %u faces, working memory was sized for %u", what, mesh->numFaces, mem->numFaces);
+    return false;
+  }
+  return true;
+}
+
+bvht_AABB LBVH_initTriData(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
+  bvht_AABB box;
+  std::memset(&box, 0, sizeof(box));
+  if (!checkMem(mem, mesh, "LBVH_initTriData")) return box;
+  Internal *in = internalOf(mem);
+  in->mesh = *mesh;
+  in->haveMesh = true;
+  k_sceneBox<<<in->boxBlocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
+                                                     in->partials, (uint32_t *)mem->sortScratch, in->plan.totalWords);
+  if (!checkLaunch("k_sceneBox")) return box;
+  BuildParams p;
+  if (check(hipMemcpyAsync(&p, mem->params, sizeof(p), hipMemcpyDeviceToHost, stream()), "LBVH_initTriData box") &&
+      check(hipStreamSynchronize(stream()), "LBVH_initTriData sync")) {
+    box.minMax[0] = {p.sceneMin[0], p.sceneMin[1], p.sceneMin[2]};
+    box.minMax[1] = {p.sceneMax[0], p.sceneMax[1], p.sceneMax[2]};
+  }
+  return box;
+}
+
+void LBVH_calcMortonCodes(bvht_LBVH_WorkingMemory *mem, bvht_AABB sceneAABB) {
+  if (!mem || !mem->lRes || !mem->internal || !internalOf(mem)->haveMesh) {
+    setError("LBVH_calcMortonCodes: call LBVH_initTriData first");
+    return;
+  }
+  Internal *in = internalOf(mem);
+  // offset / scale on the host exactly as LBVH_CUDA.cu:171-180 (f32, IEEE)
+  BuildParams p;
+  std::memset(&p, 0, sizeof(p));
+  const float mn[3] = {sceneAABB.minMax[0].x, sceneAABB.minMax[0].y, sceneAABB.minMax[0].z};
+  const float mx[3] = {sceneAABB.minMax[1].x, sceneAABB.minMax[1].y, sceneAABB.minMax[1].z};
+  for (int a = 0; a < 3; ++a) {
+    p.sceneMin[a] = mn[a];
+    p.sceneMax[a] = mx[a];
+    p.offset[a] = 0.0f - mn[a];
+    volatile float m = mx[a] + p.offset[a];
+    p.scale[a] = 1.0f / m;
+  }
+  in->hostParams = p;
+  if (!check(hipMemcpyAsync(mem->params, &in->hostParams, sizeof(p), hipMemcpyHostToDevice, stream()),
+             "LBVH_calcMortonCodes params"))
+    return;
+  if (!sortReset(in->plan, (uint32_t *)mem->sortScratch)) return;
+  launchMorton(mem, &in->mesh);
+}
+
+void LBVH_sortMortonCodes(bvht_LBVH_WorkingMemory *mem) {
+  if (!mem || !mem->lRes || !mem->internal) {
+    setError("LBVH_sortMortonCodes: working memory not allocated");
+    return;
+  }
+  runSort(mem);
+}
+
+void LBVH_buildBVHTree(bvht_LBVH_WorkingMemory *mem, bvht_CUDAMemoryBVHPointer *bvh) {
+  if (!mem || !mem->lRes || !mem->internal || !internalOf(mem)->haveMesh || !mem->sortedValues) {
+    setError("LBVH_buildBVHTree: run LBVH_initTriData, LBVH_calcMortonCodes and LBVH_sortMortonCodes first");
+    return;
+  }
+  if (!bvh || !bvh->nodes || bvh->numNodes != 2 * mem->numFaces - 1) {
+    setError("LBVH_buildBVHTree: BVH not allocated for %u faces", mem->numFaces);
+    return;
+  }
+  runTree(mem, &internalOf(mem)->mesh, bvh);
+}
+
+void LBVH_fixAABB(bvht_LBVH_WorkingMemory *mem, bvht_CUDAMemoryBVHPointer *bvh) {
+  // Full bottom-up refit from the leaves (kFixAABBTree semantics).  The
+  // build pass above already produced the same boxes; this recomputes them
+  // from the current leaves, e.g. after leaves were edited.
+  if (!mem || !mem->lRes || !bvh || !bvh->nodes) {
+    setError("LBVH_fixAABB: working memory or BVH not allocated");
+    return;
+  }
+  const uint32_t n = mem->numFaces;
+  if (n > 1) refitFromLeaves(bvh->nodes, n - 1, n, nullptr, mem->atomicLocks, mem->numLocks);
+}
+
+void LBVH_doCUDASyc(void) { check(hipStreamSynchronize(stream()), "LBVH_doCUDASyc"); }
+
+bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh) {
+  if (!checkMem(mem, mesh, "bvht_LBVH_build")) return false;
+  if (!bvh || !bvh->nodes || bvh->numNodes != 2 * mesh->numFaces - 1) {
+    setError("bvht_LBVH_build: BVH not allocated for %u faces", mesh->numFaces);
+    return false;
+  }
+  Internal *in = internalOf(mem);
+  in->mesh = *mesh;
+  in->haveMesh = true;
+  Timing &tm = timing();
+  tm.start();
+  tm.record("begin");
+  k_sceneBox<<<in->boxBlocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
+                                                     in->partials, (uint32_t *)mem->sortScratch, in->plan.totalWords);
+  if (!checkLaunch("k_sceneBox")) return false;
+  tm.record("k_sceneBox");
+  if (!launchMorton(mem, mesh)) return false;
+  tm.record("k_morton");
+  if (!runSort(mem)) return false;
+  if (!runTree(mem, mesh, bvh)) return false;
+  return true;
+}
+
+void bvht_LBVH_setTiming(bool enable) { timing().enabled = enable; }
+
+int bvht_LBVH_stageTimes(float *ms, int maxStages) {
+  Timing &tm = timing();
+  if (!tm.enabled || tm.recorded < 2) return 0;
+  if (!check(hipEventSynchronize(tm.ev[tm.recorded - 1]), "bvht_LBVH_stageTimes")) return 0;
+  int k = 0;
+  for (int i = 0; i + 1 < tm.recorded && k < maxStages; ++i, ++k) {
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, tm.ev[i], tm.ev[i + 1]);
+    ms[k] = t;
+  }
+  return k;
+}
+
+const char *bvht_LBVH_stageName(int i) {
+  Timing &tm = timing();
+  if (i < 0 || i + 1 >= tm.recorded) return "";
+  return tm.name[i + 1];
+}
+
+}  // extern "C"

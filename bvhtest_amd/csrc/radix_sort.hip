@@ -1,0 +1,211 @@
+// Onesweep radix sort kernels (see radix_sort.hpp).
+#include "radix_sort.hpp"
+#include "wave.hpp"
+
+namespace bvht {
+
+namespace {
+
+constexpr uint32_t kFlagAgg = 1u << 30;  // tile aggregate published
+constexpr uint32_t kFlagInc = 2u << 30;  // inclusive prefix published
+constexpr uint32_t kCountMask = (1u << 30) - 1;
+
+// Histogram of all passes: LDS counters per block, then one global atomic
+// per (pass, digit) per block.
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_histogram(const K *__restrict__ keys, uint32_t n, int passes,
+                                                          uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_hist[8][kSortRadix];
+  for (int p = 0; p < passes; ++p) s_hist[p][threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * kSortBlock + threadIdx.x; i < n; i += gridDim.x * kSortBlock) {
+    K k = keys[i];
+    for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p][(uint32_t)(k >> (p * kSortRadixBits)) & (kSortRadix - 1)], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p) {
+    uint32_t c = s_hist[p][threadIdx.x];
+    if (c) atomicAdd(&hist[p * kSortRadix + threadIdx.x], c);
+  }
+}
+
+// One onesweep pass.  Each block takes the next tile id from `counter`
+// (tiles start in id order, so every tile a block waits for has started),
+// ranks its keys stably with wave64 ballots, publishes per-digit counts,
+// looks back over earlier tiles for its global offsets and writes the tile
+// out digit-run by digit-run from LDS.
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
+                                                         K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
+                                                         uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
+                                                         uint32_t *__restrict__ status, uint32_t *__restrict__ counter) {
+  constexpr int WAVES = kSortBlock / 64;
+  constexpr int TILE = kSortBlock * ITEMS;
+  static_assert(kSortBlock == kSortRadix, "one digit per thread");
+  __shared__ uint32_t s_wave[WAVES][kSortRadix];
+  __shared__ uint32_t s_bstart[kSortRadix];
+  __shared__ uint32_t s_gstart[kSortRadix];
+  __shared__ uint32_t s_scan[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ K s_keys[TILE];
+  __shared__ uint32_t s_vals[TILE];
+
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  for (int w = 0; w < WAVES; ++w) s_wave[w][tid] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * TILE + wave * 64 * ITEMS;
+
+  K key[ITEMS];
+  uint32_t val[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint32_t idx = base + i * 64 + lane;
+    bool valid = idx < n;
+    key[i] = valid ? keysIn[idx] : (K)0;
+    val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
+  }
+
+  // Stable in-wave ranking: item-major order inside a wave's segment equals
+  // tile order.  Peers (same digit) are found with 8 ballots; the lowest
+  // peer bumps the wave's counter for the digit.
+  volatile uint32_t *cnt = &s_wave[wave][0];
+  const uint64_t lt = __lanemask_lt();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    bool valid = base + i * 64 + lane < n;
+    uint32_t d = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kSortRadixBits; ++b) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    uint32_t lower = (uint32_t)__popcll(peers & lt);
+    uint32_t c = cnt[d];
+    rank[i] = c + lower;
+    if (valid && lower == 0) cnt[d] = c + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+
+  // Thread tid owns digit tid: per-wave exclusive offsets and tile total.
+  uint32_t total = 0;
+  for (int w = 0; w < WAVES; ++w) {
+    uint32_t c = s_wave[w][tid];
+    s_wave[w][tid] = total;
+    total += c;
+  }
+  uint32_t *st = status + (size_t)tile * kSortRadix + tid;
+  storeRelaxedAgent(st, (tile == 0 ? kFlagInc : kFlagAgg) | total);
+  uint32_t hbase = blockExclusiveScan<kSortBlock>(hist[tid], s_scan);
+  uint32_t bstart = blockExclusiveScan<kSortBlock>(total, s_scan);
+
+  uint32_t excl = 0;
+  if (tile > 0) {
+    uint32_t t = tile - 1, spins = 0;
+    while (true) {
+      uint32_t s = loadRelaxedAgent(status + (size_t)t * kSortRadix + tid);
+      uint32_t f = s & ~kCountMask;
+      if (f == 0) {
+        // bounded wait: a predecessor that never publishes (impossible when
+        // the tile counter starts at zero) ends the pass with an error flag
+        // instead of a hang
+        if (++spins > (1u << 24)) {
+          atomicOr(counter + 8, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += s & kCountMask;
+      if (f == kFlagInc) break;
+      --t;
+    }
+    storeRelaxedAgent(st, kFlagInc | (excl + total));
+  }
+  s_bstart[tid] = bstart;
+  s_gstart[tid] = hbase + excl;
+  __syncthreads();
+
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if (base + i * 64 + lane < n) {
+      uint32_t d = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
+      uint32_t pos = s_bstart[d] + s_wave[wave][d] + rank[i];
+      s_keys[pos] = key[i];
+      s_vals[pos] = val[i];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t tileCount = min((uint32_t)TILE, n - tile * TILE);
+  for (uint32_t j = tid; j < tileCount; j += kSortBlock) {
+    K k = s_keys[j];
+    uint32_t d = (uint32_t)(k >> shift) & (kSortRadix - 1);
+    uint32_t out = s_gstart[d] + (j - s_bstart[d]);
+    if (out >= n) continue;  // only reachable after a bounded-wait error
+    if (keysOut) keysOut[out] = k;
+    valsOut[out] = s_vals[j];
+  }
+}
+
+}  // namespace
+
+SortPlan makeSortPlan(uint32_t n, int keyBytes, int keyBits) {
+  SortPlan p;
+  p.n = n;
+  p.keyBytes = keyBytes;
+  p.keyBits = keyBits;
+  p.passes = (keyBits + kSortRadixBits - 1) / kSortRadixBits;
+  p.tile = kSortBlock * (keyBytes == 8 ? kSortItems64 : kSortItems32);
+  p.numTiles = n == 0 ? 0 : divUp(n, p.tile);
+  p.histWords = (size_t)p.passes * kSortRadix;
+  p.counterWords = 16;
+  p.statusWords = (size_t)p.passes * p.numTiles * kSortRadix;
+  p.histOff = 0;
+  p.counterOff = p.histOff + p.histWords;
+  p.statusOff = p.counterOff + p.counterWords;
+  p.totalWords = p.statusOff + p.statusWords;
+  return p;
+}
+
+bool sortReset(const SortPlan &p, uint32_t *scratch) { return fillU32(scratch, 0u, p.totalWords); }
+
+bool sortHistogram(const SortPlan &p, const void *keys, uint32_t *scratch) {
+  if (p.n == 0) return true;
+  uint32_t blocks = divUp(p.n, kSortBlock * 16);
+  if (blocks > (uint32_t)numCUs() * 4) blocks = numCUs() * 4;
+  if (p.keyBytes == 8)
+    k_histogram<uint64_t><<<blocks, kSortBlock, 0, stream()>>>((const uint64_t *)keys, p.n, p.passes, scratch + p.histOff);
+  else
+    k_histogram<uint32_t><<<blocks, kSortBlock, 0, stream()>>>((const uint32_t *)keys, p.n, p.passes, scratch + p.histOff);
+  return checkLaunch("sortHistogram");
+}
+
+int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityValues, bool keepKeys, uint32_t *scratch,
+            void (*afterPass)(int)) {
+  if (p.n == 0) return 0;
+  int cur = 0;
+  for (int pass = 0; pass < p.passes; ++pass) {
+    const uint32_t *hist = scratch + p.histOff + (size_t)pass * kSortRadix;
+    uint32_t *counter = scratch + p.counterOff + pass;
+    uint32_t *status = scratch + p.statusOff + (size_t)pass * p.numTiles * kSortRadix;
+    const uint32_t *vin = (pass == 0 && identityValues) ? nullptr : vals[cur];
+    bool last = pass == p.passes - 1;
+    void *kout = (last && !keepKeys) ? nullptr : keys[cur ^ 1];
+    uint32_t shift = pass * kSortRadixBits;
+    if (p.keyBytes == 8)
+      k_onesweep<uint64_t, kSortItems64><<<p.numTiles, kSortBlock, 0, stream()>>>(
+          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+    else
+      k_onesweep<uint32_t, kSortItems32><<<p.numTiles, kSortBlock, 0, stream()>>>(
+          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+    if (!checkLaunch("onesweep pass")) return -1;
+    if (afterPass) afterPass(pass);
+    cur ^= 1;
+  }
+  return cur;
+}
+
+}  // namespace bvht

@@ -1,0 +1,45 @@
+// Generic bottom-up refit (see refit.hpp).
+//
+// Arrival counting uses agent-scope acquire/release atomics: the box a
+// thread writes for a node must be visible to the sibling's thread, which
+// may run on another XCD with its own L2.  Every walk is bounded by the node
+// count, so a malformed tree cannot hang the kernel.
+#include "refit.hpp"
+
+namespace bvht {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_refitUp(bvht_BVHNode *__restrict__ nodes, uint32_t leafFirst, uint32_t count,
+                                                 const uint32_t *__restrict__ leafList, uint32_t *__restrict__ counters,
+                                                 uint32_t numNodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t leaf = leafList ? leafList[i] : leafFirst + i;
+  uint32_t node = nodes[leaf].parent;
+  if (node == leaf || node >= numNodes) return;
+  for (uint32_t steps = 0; steps < numNodes; ++steps) {
+    if (__hip_atomic_fetch_add(counters + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    const uint32_t l = nodes[node].left, r = nodes[node].right;
+    const bvht_AABB a = nodes[l].bbox, b = nodes[r].bbox;
+    bvht_AABB m = a;
+    mergeInto(m, b);
+    nodes[node].bbox = m;
+    nodes[node].surfaceArea = surfaceArea(m);
+    const uint32_t parent = nodes[node].parent;
+    if (parent == node || parent >= numNodes) return;
+    node = parent;
+  }
+}
+
+}  // namespace
+
+bool refitFromLeaves(bvht_BVHNode *nodes, uint32_t leafFirst, uint32_t count, const uint32_t *leafList,
+                     uint32_t *counters, uint32_t numCounters) {
+  if (count == 0) return true;
+  k_refitUp<<<divUp(count, 256), 256, 0, stream()>>>(nodes, leafFirst, count, leafList, counters, numCounters);
+  if (!checkLaunch("k_refitUp")) return false;
+  return fillU32(counters, 0u, numCounters);
+}
+
+}  // namespace bvht

@@ -1,0 +1,65 @@
+// wave64 / workgroup primitives shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bvht {
+
+// Device-coherent (all XCDs) relaxed accesses.  On gfx950 these compile to
+// loads/stores with the sc1 cache-coherence bit and no cache maintenance;
+// use them for self-contained words (flag + payload in one u32).
+__device__ __forceinline__ void storeRelaxedAgent(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t loadRelaxedAgent(const uint32_t *p) {
+  return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inclusive scan across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t x) {
+  const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan across a workgroup of BLOCK threads (BLOCK multiple of 64).
+// s_tmp needs BLOCK / 64 words.  Contains barriers: call from uniform code.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t *s_tmp) {
+  constexpr int WAVES = BLOCK / 64;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = waveInclusiveScan(v);
+  if (lane == 63) s_tmp[wave] = inc;
+  __syncthreads();
+  uint32_t prefix = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w)
+    if ((uint32_t)w < wave) prefix += s_tmp[w];
+  __syncthreads();
+  return prefix + inc - v;
+}
+
+__device__ __forceinline__ float waveMin(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float y = __shfl_xor(x, o, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+__device__ __forceinline__ float waveMax(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float y = __shfl_xor(x, o, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+}  // namespace bvht

@@ -1,0 +1,369 @@
+/*
+ * bvhtest_amd.h -- C ABI of the MI355X (gfx950) BVH pipeline.
+ *
+ * Every entry point below is `extern "C"`, takes plain pointers, sizes and
+ * plain-old-data structs, and has no C++ or torch types in its signature.
+ * Each declaration cites the reference interface it replaces
+ * (mensinda/BVHTest, paths relative to the reference repository root).
+ *
+ * Layouts: the POD structs reproduce the reference's memory layout byte for
+ * byte (BVHNode 48 B, AABB 24 B, Triangle 12 B, Ray 48 B, BVH header 32 B),
+ * so buffers written by this library can be handed to reference code and
+ * vice versa.  Static asserts in bvhtest_amd/csrc/types.hpp pin the offsets.
+ *
+ * Error convention (reference: `CUDA_RUN` macros, lib/builder/LBVH_CUDA.cu:29-34):
+ * functions return what the reference returns (bool / struct with a result
+ * flag / void).  Instead of printing, the library records the last failure;
+ * read it with bvht_getLastError().  Kernel launches are checked.
+ *
+ * Threading: like the reference (single host thread, one stream), calls are
+ * not re-entrant.  All device work is issued on the stream set with
+ * bvht_setStream() (default: the null stream).
+ */
+#ifndef BVHTEST_AMD_H
+#define BVHTEST_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Every declaration below is exported; the library hides everything else. */
+#pragma GCC visibility push(default)
+
+/* ------------------------------------------------------------------------ */
+/* Data model (reference: lib/base/BVH.hpp:38-238, lib/base/Ray.hpp:38-80)   */
+/* ------------------------------------------------------------------------ */
+
+/* glm::vec3 (3 x f32, align 4) */
+typedef struct bvht_vec3 {
+  float x, y, z;
+} bvht_vec3;
+
+/* AABB {vec3 minMax[2]} -- lib/base/BVH.hpp:59-60 */
+typedef struct bvht_AABB {
+  bvht_vec3 minMax[2]; /* 0: min, 1: max */
+} bvht_AABB;
+
+/* Triangle {u32 v1, v2, v3} -- lib/base/BVH.hpp:38-42 */
+typedef struct bvht_Triangle {
+  uint32_t v1, v2, v3;
+} bvht_Triangle;
+
+/* BVHNode, alignas(16), 48 B -- lib/base/BVH.hpp:113-123.
+ * Leaves: left = first face index, right = face count (1 for LBVH). */
+typedef struct __attribute__((aligned(16))) bvht_BVHNode {
+  bvht_AABB bbox;       /* @0  */
+  uint32_t parent;      /* @24 */
+  uint32_t left;        /* @28 */
+  uint32_t right;       /* @32 */
+  uint16_t isLeafFlag;  /* @36 */
+  uint16_t isLeft;      /* @38 */
+  uint16_t level;       /* @40 */
+  uint16_t _pad;        /* @42 (padding in the reference) */
+  float surfaceArea;    /* @44 */
+} bvht_BVHNode;
+
+/* Device-side BVH header, 32 B -- lib/base/BVH.hpp:138-144 (class BVH data
+ * members: node pointer, size, capacity, root index, max level). */
+typedef struct bvht_BVH {
+  bvht_BVHNode *bvh;
+  size_t vSize;
+  size_t vCapacity;
+  uint32_t vRootIndex;
+  uint16_t vMaxLevel;
+  uint16_t _pad;
+} bvht_BVH;
+
+/* MeshRaw -- lib/base/BVH.hpp:50-57 (device pointers + counts) */
+typedef struct bvht_MeshRaw {
+  bvht_vec3 *vert;
+  bvht_vec3 *norm;
+  bvht_Triangle *faces;
+  uint32_t numVert;
+  uint32_t numNorm;
+  uint32_t numFaces;
+} bvht_MeshRaw;
+
+/* CUDAMemoryBVHPointer -- lib/base/BVH.hpp:234-238 */
+typedef struct bvht_CUDAMemoryBVHPointer {
+  bvht_BVH *bvh;        /* device header */
+  bvht_BVHNode *nodes;  /* device nodes  */
+  uint32_t numNodes;
+} bvht_CUDAMemoryBVHPointer;
+
+/* Ray, alignas(16), 48 B -- lib/base/Ray.hpp:38-53 */
+typedef struct __attribute__((aligned(16))) bvht_Ray {
+  bvht_vec3 pos;     /* @0  */
+  bvht_vec3 dir;     /* @12 */
+  bvht_vec3 invDir;  /* @24 */
+  uint8_t sign[3];   /* @36 (struct Sign, align 4) */
+  uint8_t _pad;
+  uint32_t px;       /* @40 */
+  uint32_t py;       /* @44 */
+} bvht_Ray;
+
+/* CUDAPixel {u8 hit, u8 diffuse, u16 intCount} -- lib/tracer/CUDAKernels.hpp:33-37 */
+typedef struct bvht_CUDAPixel {
+  uint8_t hit;
+  uint8_t diffuse;
+  uint16_t intCount;
+} bvht_CUDAPixel;
+
+/* Hit record (extension: the reference writes no triangle id or distance,
+ * lib/tracer/CUDAKernels.cu:208-220).  triId = UINT32_MAX on a miss. */
+typedef struct bvht_Hit {
+  uint32_t triId;
+  float t;
+} bvht_Hit;
+
+/* ------------------------------------------------------------------------ */
+/* Library control (no reference counterpart: the reference uses the legacy  */
+/* default stream and prints errors)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Stream every later call issues its work on (hipStream_t; NULL = default). */
+void bvht_setStream(void *stream);
+void *bvht_getStream(void);
+/* Message of the last failure ("" if none) and reset of it. */
+const char *bvht_getLastError(void);
+void bvht_clearError(void);
+/* Host-test hook: make the n-th device allocation from now fail (n >= 1);
+ * 0 disables.  Exercises the CUDA_ERROR paths, e.g. lib/builder/LBVH.cpp:29-35. */
+void bvht_setAllocFailAfter(int n);
+/* Library version string and the gfx target it was built for. */
+const char *bvht_version(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int bvht_deviceCount(void);
+
+/* ------------------------------------------------------------------------ */
+/* Memory and copies -- lib/cuda/cudaFN.hpp:27-42 (extern "C" in reference)  */
+/* ------------------------------------------------------------------------ */
+
+/* runMalloc/runMemcpy/runFree -- lib/cuda/cudaFN.hpp:37-40.
+ * runMemcpy kinds: 0 host->host, 1 host->device, 2 device->host,
+ * 3 device->device (hipMemcpyKind values). */
+bool runMalloc(void **ptr, size_t size);
+bool runMemcpy(void *dest, const void *src, size_t size, int kind);
+void runFree(void *ptr);
+
+/* copyBVHToGPU -- lib/cuda/cudaFN.cu:32-74.  Host nodes + root/maxLevel in,
+ * device header + nodes out.  numNodes == 0 gives an empty device BVH. */
+bool copyBVHToGPU(const bvht_BVHNode *hostNodes, uint32_t numNodes, uint32_t root, uint16_t maxLevel,
+                  bvht_CUDAMemoryBVHPointer *out);
+/* copyMeshToGPU -- lib/cuda/cudaFN.cu:76-109. */
+bool copyMeshToGPU(const bvht_vec3 *vert, uint32_t numVert, const bvht_vec3 *norm, uint32_t numNorm,
+                   const bvht_Triangle *faces, uint32_t numFaces, bvht_MeshRaw *out);
+/* copyBVHToHost -- lib/cuda/cudaFN.cu:113-146.  Reads the device header for
+ * root/size/maxLevel, copies nodes into hostNodes (capacity nodes) and frees
+ * the device BVH like the reference.  numNodes, root and maxLevel (optional)
+ * receive the header fields. */
+bool copyBVHToHost(bvht_CUDAMemoryBVHPointer *bvh, bvht_BVHNode *hostNodes, uint32_t capacity,
+                   uint32_t *numNodes, uint32_t *root, uint16_t *maxLevel);
+/* copyMeshToHost -- lib/cuda/cudaFN.cu:149-182 (copies, then frees the device mesh). */
+bool copyMeshToHost(bvht_MeshRaw *mesh, bvht_vec3 *vert, bvht_vec3 *norm, bvht_Triangle *faces);
+/* resetCUDA -- lib/cuda/cudaFN.cu:184-187 (here: device sync + release of
+ * library-owned scratch; no hipDeviceReset, which would invalidate the
+ * caller's other allocations). */
+void resetCUDA(void);
+/* Free a device BVH / mesh without copying (no reference counterpart; the
+ * reference frees only inside copy*ToHost). */
+void bvht_freeBVH(bvht_CUDAMemoryBVHPointer *bvh);
+void bvht_freeMesh(bvht_MeshRaw *mesh);
+
+/* ------------------------------------------------------------------------ */
+/* LBVH builder -- lib/builder/LBVH_CUDA.hpp:27-59 (C++ linkage in reference) */
+/* ------------------------------------------------------------------------ */
+
+/* Build options (extension).  Defaults (bvht_LBVH_defaultOptions) reproduce
+ * the reference bit for bit: 30-bit Morton keys, index-only topology. */
+typedef struct bvht_LBVH_Options {
+  uint32_t mortonBits;   /* 30 (reference, lib/builder/LBVH_CUDA.cu:130-166) or 63 (21 bits/axis) */
+  uint32_t topology;     /* 0: reference topology (delta = clz32 of the low word, i.e. of the
+                            sorted index, lib/builder/LBVH_CUDA.cu:205-212); 1: Karras 2012 over
+                            the full 64-bit key (morton<<32 | index) */
+  uint32_t tileLog2;     /* log2 of leaves per build workgroup (8..11), 0 = default */
+  uint32_t digitBits;    /* radix digit width of the sort (8 or 11... ), 0 = default */
+} bvht_LBVH_Options;
+
+/* LBVH_WorkingMemory -- lib/builder/LBVH_CUDA.hpp:37-47 (scratch handles). */
+typedef struct bvht_LBVH_WorkingMemory {
+  bool lRes;
+  bvht_LBVH_Options opts;
+  uint32_t numFaces;
+  void *keys[2];          /* ping-pong sort keys (u32 or u64) */
+  uint32_t *values[2];    /* ping-pong face indices */
+  uint32_t *sortedValues; /* -> face index of leaf i after LBVH_sortMortonCodes */
+  void *sortedKeys;       /* -> Morton key of leaf i (NULL if the sort dropped keys) */
+  void *sortScratch;      /* histograms, look-back state, tile counters */
+  size_t sortScratchBytes;
+  void *params;           /* device: scene box, offset, scale, counters */
+  uint32_t *atomicLocks;  /* refit arrival counters */
+  uint32_t numLocks;
+  void *internal;         /* host-side bookkeeping */
+} bvht_LBVH_WorkingMemory;
+
+bvht_LBVH_Options bvht_LBVH_defaultOptions(void);
+
+/* LBVH_allocateBVH -- lib/builder/LBVH_CUDA.cu:396-424.  Fails if *bvh already
+ * holds memory.  Writes the device header (root 0, size 2n-1). */
+bool LBVH_allocateBVH(bvht_CUDAMemoryBVHPointer *bvh, const bvht_MeshRaw *mesh);
+/* LBVH_allocateWorkingMemory -- lib/builder/LBVH_CUDA.cu:426-475 (reference options). */
+bvht_LBVH_WorkingMemory LBVH_allocateWorkingMemory(const bvht_MeshRaw *mesh);
+/* Same with explicit options (extension). */
+bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *mesh, const bvht_LBVH_Options *opts);
+/* LBVH_initTriData -- lib/builder/LBVH_CUDA.cu:98-119: scene box = union of the
+ * per-triangle boxes (+-FLT_EPSILON) and the origin.  Synchronises and returns
+ * the box like the reference (the fused build below does not). */
+bvht_AABB LBVH_initTriData(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh);
+/* LBVH_calcMortonCodes -- lib/builder/LBVH_CUDA.cu:168-183 (offset/scale from the box). */
+void LBVH_calcMortonCodes(bvht_LBVH_WorkingMemory *mem, bvht_AABB sceneAABB);
+/* LBVH_sortMortonCodes -- lib/builder/LBVH_CUDA.cu:185-193 (stable LSD radix sort). */
+void LBVH_sortMortonCodes(bvht_LBVH_WorkingMemory *mem);
+/* LBVH_buildBVHTree -- lib/builder/LBVH_CUDA.cu:326-330: leaves + Karras
+ * topology.  Here the same pass also refits every node below the top tiles. */
+void LBVH_buildBVHTree(bvht_LBVH_WorkingMemory *mem, bvht_CUDAMemoryBVHPointer *bvh);
+/* LBVH_fixAABB -- lib/builder/LBVH_CUDA.cu:376-385: completes the bottom-up refit. */
+void LBVH_fixAABB(bvht_LBVH_WorkingMemory *mem, bvht_CUDAMemoryBVHPointer *bvh);
+/* LBVH_freeWorkingMemory -- lib/builder/LBVH_CUDA.cu:477-488. */
+void LBVH_freeWorkingMemory(bvht_LBVH_WorkingMemory *mem);
+/* LBVH_doCUDASyc -- lib/builder/LBVH_CUDA.cu:490 (stream sync). */
+void LBVH_doCUDASyc(void);
+/* Whole build on the stream without host round trips (extension: the scene
+ * box stays on the device).  Same result as the five staged calls above. */
+bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh);
+/* Per-kernel device times (ms, HIP events on the library stream) of the last
+ * bvht_LBVH_build when timing is on; bvht_LBVH_stageName(i) names kernel i
+ * (k_sceneBox, k_morton, k_onesweep per pass, k_buildLevel0, k_buildLevelUp).
+ * Returns the number of entries written. */
+void bvht_LBVH_setTiming(bool enable);
+int bvht_LBVH_stageTimes(float *ms, int maxStages);
+const char *bvht_LBVH_stageName(int i);
+
+/* ------------------------------------------------------------------------ */
+/* Bittner13 GPU optimizer -- lib/builder/Bittner13CUDA.hpp:56-83           */
+/* ------------------------------------------------------------------------ */
+
+/* AlgoCFG -- lib/builder/Bittner13CUDA.hpp:56-66 (+ extension fields). */
+typedef struct bvht_AlgoCFG {
+  uint32_t numChunks;
+  uint32_t chunkSize;
+  uint32_t blockSize;
+  bool offsetAccess;
+  bool altFindNode;
+  bool altFixTree;
+  bool altSort;
+  bool sort;
+  bool localPatchCPY;
+  /* extension */
+  uint32_t priority;       /* 0: reference heuristic (kCalcCost), 1: learned MLP */
+  uint32_t lanesPerSearch; /* lanes cooperating on one insertion search (0 = default) */
+} bvht_AlgoCFG;
+
+/* GPUWorkingMemory -- lib/builder/Bittner13CUDA.hpp:39-54 (opaque here). */
+typedef struct bvht_GPUWorkingMemory {
+  bool result;
+  uint32_t numLeafNodes;
+  uint32_t numInnerNodes;
+  uint32_t numPatches;
+  uint32_t numSkipped;
+  uint32_t numNodesToFix;
+  uint32_t numFlags;
+  void *internal;
+} bvht_GPUWorkingMemory;
+
+/* allocateMemory -- lib/builder/Bittner13CUDA.cu:836-919 */
+bvht_GPUWorkingMemory allocateMemory(bvht_CUDAMemoryBVHPointer *bvh, uint32_t batchSize, uint32_t numFaces);
+/* freeMemory -- lib/builder/Bittner13CUDA.cu:921-936 */
+void freeMemory(bvht_GPUWorkingMemory *data);
+/* initData -- lib/builder/Bittner13CUDA.cu:939-987 */
+void initData(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t blockSize);
+/* fixTree1 -- lib/builder/Bittner13CUDA.cu:641-648 (full bottom-up refit) */
+void fixTree1(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t blockSize);
+/* fixTree3 -- lib/builder/Bittner13CUDA.cu:650-656 (refit of the touched paths) */
+void fixTree3(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t blockSize);
+/* bn13_selectNodes -- lib/builder/Bittner13CUDA.cu:659-726 */
+void bn13_selectNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg);
+/* bn13_rmAndReinsChunk -- lib/builder/Bittner13CUDA.cu:729-766 */
+void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg,
+                          uint32_t chunk);
+/* bn13_doAlgorithmStep -- lib/builder/Bittner13CUDA.cu:768-772 */
+void bn13_doAlgorithmStep(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg);
+/* CUDAcalcSAH -- lib/builder/Bittner13CUDA.cu:776-803: (sum SA - SA(root)) / SA(root) */
+float CUDAcalcSAH(bvht_CUDAMemoryBVHPointer *bvh);
+/* doCudaDevSync -- lib/builder/Bittner13CUDA.cu:824 */
+void doCudaDevSync(void);
+/* calcNumSkipped -- lib/builder/Bittner13CUDA.cu:806-821 */
+uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data);
+/* Optimizer counters since initData (extension; no reference counterpart):
+ * [0] lock conflicts, [1] patches dropped (removal impossible or reinsertion
+ * at the root), [2] patches dropped because their insertion point lies in a
+ * subtree another patch moves, [3] search frontier overflows, [4] applied
+ * patches.  Returns the number of words written. */
+int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords);
+/* Weighted SAH of the device tree (lib/base/BVH.cpp:23-37 semantics:
+ * sum over non-root nodes of SA * (leaf ? cLeaf : cInner) / SA(root)),
+ * accumulated in f64 (extension). */
+double bvht_calcSAHWeighted(bvht_CUDAMemoryBVHPointer *bvh, float cInner, float cLeaf);
+
+/* ------------------------------------------------------------------------ */
+/* Ray tracer -- lib/tracer/CUDAKernels.hpp:39-78 (extern "C" in reference)  */
+/* ------------------------------------------------------------------------ */
+
+bool allocateRays(bvht_Ray **rays, uint32_t num);                 /* CUDAKernels.hpp:39 */
+bool allocateImage(bvht_CUDAPixel **img, uint32_t w, uint32_t h); /* CUDAKernels.hpp:40 */
+void freeRays(bvht_Ray **rays);                                   /* CUDAKernels.hpp:41 */
+void freeImage(bvht_CUDAPixel *img);                              /* CUDAKernels.hpp:42 */
+/* generateRays -- lib/tracer/CUDAKernels.cu:386-397 (pinhole camera). */
+void generateRays(bvht_Ray *rays, uint32_t w, uint32_t h, bvht_vec3 pos, bvht_vec3 lookAt, bvht_vec3 up, float fov);
+/* tracerImage -- lib/tracer/CUDAKernels.cu:399-417.  bundle selects the
+ * wave-coherent traversal (reference kTraceRayBundle). */
+void tracerImage(bvht_Ray *rays, bvht_CUDAPixel *img, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
+                 bvht_vec3 light, uint32_t w, uint32_t h, bool bundle);
+/* copyImageToHost -- CUDAKernels.hpp:67 */
+void copyImageToHost(bvht_CUDAPixel *hostPixel, bvht_CUDAPixel *cudaImg, uint32_t w, uint32_t h);
+/* tracerDoCudaSync -- CUDAKernels.hpp:77 */
+void tracerDoCudaSync(void);
+
+/* Extension: trace n rays with the same traversal and write per-ray hits
+ * (and the reference pixel when img != NULL).  mode 0 = reference triangle
+ * test (no t-min, CUDAKernels.cu:88-112); mode 1 = t > tMin and the triangle
+ * `skipTri[i]` (if skipTri != NULL) is ignored, for secondary rays. */
+bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
+                    bvht_vec3 light, int mode, float tMin, const uint32_t *skipTri, bvht_Hit *hits,
+                    bvht_CUDAPixel *img);
+/* Extension: one cosine-weighted diffuse bounce per primary hit: builds
+ * secondary rays (origin hit + 1e-4 * n, n facing the incoming ray, direction
+ * from a per-pixel hash of (seed, pixel)) into outRays; rays of missed pixels
+ * get dir = 0 and are skipped by bvht_traceRays (hit = miss). */
+bool bvht_makeBounceRays(const bvht_Ray *primary, const bvht_Hit *hits, uint32_t num, bvht_MeshRaw mesh,
+                         uint32_t seed, bvht_Ray *outRays, uint32_t *outSkipTri);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic scenes (host only; no reference counterpart -- the reference    */
+/* loads meshes from disk with lib/io/Load.cpp)                              */
+/* ------------------------------------------------------------------------ */
+
+/* Heightfield gx x gz cells, 2 triangles per cell, heights from seeded
+ * 4-octave value noise in [0, amplitude].  Returns counts when buffers are NULL. */
+void bvht_sceneHeightfield(uint32_t gx, uint32_t gz, uint32_t seed, float amplitude, bvht_vec3 *vert,
+                           bvht_Triangle *faces, uint32_t *numVert, uint32_t *numFaces);
+/* Icosphere clusters + triangle soup (the 4M-triangle scene).  Counts first
+ * (buffers NULL), then fill. */
+void bvht_sceneSpheresSoup(uint32_t numSpheres, uint32_t numSoup, uint32_t seed, bvht_vec3 *vert,
+                           bvht_Triangle *faces, uint32_t *numVert, uint32_t *numFaces);
+/* Uniform triangle soup in the unit cube. */
+void bvht_sceneSoup(uint32_t numTris, uint32_t seed, float size, bvht_vec3 *vert, bvht_Triangle *faces,
+                    uint32_t *numVert, uint32_t *numFaces);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BVHTEST_AMD_H */

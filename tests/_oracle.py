@@ -1,0 +1,160 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: the oracle is the checker, never the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+NODE_DTYPE = np.dtype([("bbox", "<f4", (6,)), ("parent", "<u4"), ("left", "<u4"), ("right", "<u4"),
+                       ("isLeafFlag", "<u2"), ("isLeft", "<u2"), ("level", "<u2"), ("pad", "<u2"),
+                       ("surfaceArea", "<f4")])
+RAY_DTYPE = np.dtype([("pos", "<f4", (3,)), ("dir", "<f4", (3,)), ("invDir", "<f4", (3,)),
+                      ("sign", "u1", (3,)), ("pad", "u1"), ("px", "<u4"), ("py", "<u4")])
+PIXEL_DTYPE = np.dtype([("hit", "u1"), ("diffuse", "u1"), ("intCount", "<u2")])
+HIT_DTYPE = np.dtype([("triId", "<u4"), ("t", "<f4")])
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_f = ctypes.c_float
+
+_SIG = {
+    "orc_lbvh": (ctypes.c_int, [_vp, _u32, _vp, _u32, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "orc_spread3": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_int]),
+    "orc_init_tri": (None, [_vp, _vp, _u32, _vp, _vp]),
+    "orc_box_sa": (_f, [_vp]),
+    "orc_calc_sah": (ctypes.c_double, [_vp, _u32, _u32, _f, _f]),
+    "orc_fix_levels": (_u32, [_vp, _u32, _u32]),
+    "orc_validate": (_u32, [_vp, _u32, _u32, _u32, _vp, _vp, _u32, _vp]),
+    "orc_gen_rays": (None, [_u32, _u32, _vp, _vp, _vp, _f, _vp]),
+    "orc_set_ray": (None, [_vp, _vp, _vp, _u32, _u32]),
+    "orc_trace": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f, _vp, _vp, _vp]),
+    "orc_trace_bruteforce": (None, [_vp, _u32, _vp, _vp, _u32, ctypes.c_int, _f, _vp]),
+    "orc_bittner13_seq": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(_u32), _f, _u32, ctypes.c_int, _vp, _vp]),
+    "orc_bittner13_gpuref": (ctypes.c_int, [_vp, _u32, _f, _u32, _u32, ctypes.c_int, _vp, _vp]),
+    "orc_tree_is_valid": (ctypes.c_int, [_vp, _u32, _u32]),
+    "orc_seq_find_node_box": (_u32, [_vp, _u32, _u32, _vp, _f]),
+    "orc_lbvh_omp": (ctypes.c_double, [_vp, _vp, _u32, ctypes.c_int, _vp, _vp, ctypes.c_int]),
+    "orc_omp_max_threads": (ctypes.c_int, []),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"oracle not built: {ORACLE_PATH} (make -C oracle)")
+        L = ctypes.CDLL(ORACLE_PATH)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def lbvh(vert, faces, morton_bits=30, topology=0):
+    """Returns (sorted_keys u64, sorted_faces u32, nodes, scene[12])."""
+    vert = np.ascontiguousarray(vert, np.float32)
+    faces = np.ascontiguousarray(faces, np.uint32)
+    n = len(faces)
+    keys = np.empty(n, np.uint64)
+    sf = np.empty(n, np.uint32)
+    nodes = np.zeros(2 * n - 1, NODE_DTYPE)
+    scene = np.empty(12, np.float32)
+    rc = lib().orc_lbvh(p(vert), len(vert), p(faces), n, morton_bits, topology, p(keys), p(sf), p(nodes), p(scene))
+    assert rc == 0
+    return keys, sf, nodes, scene
+
+
+def fix_levels(nodes, root=0):
+    return lib().orc_fix_levels(p(nodes), len(nodes), root)
+
+
+def validate(nodes, root, max_level, vert, faces):
+    err = np.zeros(6, np.uint32)
+    vert = np.ascontiguousarray(vert, np.float32)
+    faces = np.ascontiguousarray(faces, np.uint32)
+    total = lib().orc_validate(p(nodes), len(nodes), root, max_level, p(vert), p(faces), len(faces), p(err))
+    return total, err
+
+
+def calc_sah(nodes, root=0, c_inner=1.0, c_leaf=1.0):
+    return lib().orc_calc_sah(p(nodes), len(nodes), root, c_inner, c_leaf)
+
+
+def gen_rays(w, h, pos, look_at, up, fov):
+    rays = np.empty(w * h, RAY_DTYPE)
+    f3 = lambda x: np.ascontiguousarray(x, np.float32)
+    pos, look_at, up = f3(pos), f3(look_at), f3(up)
+    lib().orc_gen_rays(w, h, p(pos), p(look_at), p(up), fov, p(rays))
+    return rays
+
+
+def trace(rays, nodes, root, vert, faces, light=(2, 2, 2), culling=True, mode=0, tmin=0.0, skip=None):
+    n = len(rays)
+    pix = np.empty(n, PIXEL_DTYPE)
+    hits = np.empty(n, HIT_DTYPE)
+    light = np.ascontiguousarray(light, np.float32)
+    vert = np.ascontiguousarray(vert, np.float32)
+    faces = np.ascontiguousarray(faces, np.uint32)
+    skip = None if skip is None else np.ascontiguousarray(skip, np.uint32)
+    lib().orc_trace(p(rays), n, p(nodes), root, p(vert), p(faces), p(light), int(culling), mode, tmin, p(skip),
+                    p(pix), p(hits))
+    return pix, hits
+
+
+def trace_bruteforce(rays, vert, faces, culling=True, tmin=0.0):
+    hits = np.empty(len(rays), HIT_DTYPE)
+    vert = np.ascontiguousarray(vert, np.float32)
+    faces = np.ascontiguousarray(faces, np.uint32)
+    lib().orc_trace_bruteforce(p(rays), len(rays), p(vert), p(faces), len(faces), int(culling), tmin, p(hits))
+    return hits
+
+
+def bittner_seq(nodes, root, batch_percent, steps, sort=True):
+    nodes = nodes.copy()
+    r = _u32(root)
+    sah = np.empty(steps + 1, np.float64)
+    ov = np.zeros(1, np.uint64)
+    rc = lib().orc_bittner13_seq(p(nodes), len(nodes), ctypes.byref(r), batch_percent, steps, int(sort), p(sah), p(ov))
+    assert rc == 0
+    return nodes, r.value, sah
+
+
+def bittner_gpuref(nodes, batch_percent, num_chunks, steps, offset_access=False):
+    nodes = nodes.copy()
+    sah = np.empty(steps + 1, np.float64)
+    stats = np.zeros(4, np.uint64)
+    rc = lib().orc_bittner13_gpuref(p(nodes), len(nodes), batch_percent, num_chunks, steps, int(offset_access),
+                                    p(sah), p(stats))
+    return rc, nodes, sah, stats
+
+
+def tree_is_valid(nodes, root=0):
+    return bool(lib().orc_tree_is_valid(p(nodes), len(nodes), root))
+
+
+def lbvh_omp(vert, faces, morton_bits=30, threads=0):
+    vert = np.ascontiguousarray(vert, np.float32)
+    faces = np.ascontiguousarray(faces, np.uint32)
+    n = len(faces)
+    nodes = np.zeros(2 * n - 1, NODE_DTYPE)
+    sf = np.empty(n, np.uint32)
+    secs = lib().orc_lbvh_omp(p(vert), p(faces), n, morton_bits, p(nodes), p(sf), threads)
+    return secs, nodes, sf
+
+
+def seq_find_node_box(nodes, root, box6, sa):
+    box6 = np.ascontiguousarray(box6, np.float32)
+    return lib().orc_seq_find_node_box(p(nodes), len(nodes), root, p(box6), sa)

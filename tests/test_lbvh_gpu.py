@@ -1,0 +1,139 @@
+"""GPU parity of the LBVH builder against the oracle, through the C ABI.
+
+Bar: bit-exact.  Morton keys, sorted permutation, and every byte of every
+48-byte node (topology, boxes, SA, flags) must equal the oracle's
+restatement of lib/builder/LBVH_CUDA.cu.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def soup(n, seed=0, spread=0.02):
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3), dtype=np.float32)
+    v = (c[:, None, :] + (rng.random((n, 3, 3), dtype=np.float32) - 0.5) * spread).reshape(-1, 3)
+    return v.astype(np.float32), np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
+def meshes(bvh):
+    out = {f"soup{n}": soup(n, seed=n) for n in (1, 2, 3, 17, 64, 1000, 1025, 4097, 100003)}
+    out["heightfield65536"] = bvh.scene_heightfield(256, 128, 1, 0.2)
+    out["spheres_small"] = bvh.scene_spheres_soup(64, 5000, 3)
+    # shared vertices and degenerate / coincident triangles
+    v, f = bvh.scene_heightfield(40, 20, 4, 0.0)
+    out["flat_dupes"] = (v, np.concatenate([f, f[:100]]))
+    return out
+
+
+@pytest.fixture(scope="module")
+def cases(bvh):
+    return meshes(bvh)
+
+
+def nodes_equal(a, b):
+    if a.tobytes() == b.tobytes():
+        return True
+    for name in a.dtype.names:
+        if not np.array_equal(a[name], b[name]):
+            bad = np.nonzero((a[name] != b[name]).reshape(len(a), -1).any(axis=1))[0]
+            print(f"field {name} differs at {len(bad)} nodes, first {bad[:5]}: {a[bad[:3]]} vs {b[bad[:3]]}")
+    return False
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("name", ["soup1", "soup2", "soup3", "soup17", "soup64", "soup1000", "soup1025", "soup4097",
+                                  "soup100003", "heightfield65536", "spheres_small", "flat_dupes"])
+def test_fused_build_bit_exact(bvh, orc, cases, name, bits):
+    v, f = cases[name]
+    print(f"case {name} n={len(f)} bits={bits}", flush=True)
+    keys, sf, expect, scene = orc.lbvh(v, f, bits)
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, bits)
+    tree = b.allocate_bvh()
+    b.build(tree)
+    got_sf = b.sorted_faces()
+    got = tree.download()
+    assert np.array_equal(got_sf, sf)
+    assert nodes_equal(got, expect)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("name", ["soup3", "soup1025", "heightfield65536", "spheres_small"])
+def test_staged_build_bit_exact(bvh, orc, cases, name, bits):
+    v, f = cases[name]
+    print(f"case {name} n={len(f)} bits={bits}", flush=True)
+    keys, sf, expect, scene = orc.lbvh(v, f, bits)
+    unsorted = np.empty_like(keys)
+    unsorted[sf] = keys
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, bits)
+    tree = b.allocate_bvh()
+    got_keys = []
+    box = b.build_staged(tree, keys_out=got_keys)
+    assert np.array_equal(box.reshape(-1), scene[:6])
+    assert np.array_equal(got_keys[0].astype(np.uint64), unsorted)
+    assert np.array_equal(b.sorted_faces(), sf)
+    assert nodes_equal(tree.download(), expect)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("name", ["soup17", "soup4097", "heightfield65536", "spheres_small", "flat_dupes"])
+def test_karras_topology_extension(bvh, orc, cases, name, bits):
+    """topology 1: Karras 2012 over the full (morton, index) key."""
+    v, f = cases[name]
+    print(f"case {name} n={len(f)} bits={bits}", flush=True)
+    keys, sf, expect, _ = orc.lbvh(v, f, bits, topology=1)
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, bits, topology=1)
+    tree = b.allocate_bvh()
+    b.build(tree)
+    assert np.array_equal(b.sorted_keys().astype(np.uint64), keys)
+    got = tree.download()
+    # parent of leaves is written by the topology kernel; the reference's
+    # kGenLeafNodes UINT32_MAX is overwritten for n > 1
+    assert nodes_equal(got, expect)
+
+
+def test_copy_to_host_and_validate(bvh, orc):
+    v, f = bvh.scene_spheres_soup(128, 20000, 9)
+    mesh = bvh.DeviceMesh(v, f)
+    tree = bvh.lbvh_build(mesh)
+    nodes, root, ml = tree.to_host()
+    assert root == 0 and len(nodes) == 2 * len(f) - 1
+    max_level = orc.fix_levels(nodes, root)
+    total, err = orc.validate(nodes, root, max_level, v, f)
+    assert total == 0, err
+
+
+def test_full_size_63bit_scene(bvh, orc):
+    """The ~4M-triangle scene at full size: bit-exact against the oracle."""
+    v, f = bvh.scene_spheres_soup(2048, 500000, 2)
+    print(f"full scene n={len(f)}", flush=True)
+    keys, sf, expect, _ = orc.lbvh(v, f, 63)
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, 63)
+    tree = b.allocate_bvh()
+    for _ in range(2):  # a rebuild into the same memory gives the same tree
+        b.build(tree)
+        assert np.array_equal(b.sorted_faces(), sf)
+        assert tree.download().tobytes() == expect.tobytes()
+
+
+def test_errors_are_reported(bvh):
+    L = bvh.lib()
+    v, f = soup(10)
+    mesh = bvh.DeviceMesh(v, f)
+    with pytest.raises(bvh.BVHTestError, match="mortonBits"):
+        bvh.LBVHBuilder(mesh, 31)
+    L.bvht_setAllocFailAfter(2)
+    with pytest.raises(bvh.BVHTestError, match="injected allocation failure"):
+        bvh.LBVHBuilder(mesh, 30)
+    L.bvht_setAllocFailAfter(0)
+    b = bvh.LBVHBuilder(mesh, 30)
+    tree = b.allocate_bvh()
+    with pytest.raises(bvh.BVHTestError, match="already allocated"):
+        import ctypes
+        if not L.LBVH_allocateBVH(ctypes.byref(tree.ptr), ctypes.byref(mesh.raw)):
+            bvh._fail("LBVH_allocateBVH")

@@ -1,0 +1,91 @@
+"""GPU parity of ray generation and traversal against the oracle.
+
+Bar: bit-exact.  Rays (f64 camera arithmetic rounded once), and per ray the
+hit triangle id, hit distance, hit flag, diffuse value and inner-node count
+of the reference's stackless traversal (lib/tracer/CUDAKernels.cu:114-223)
+must equal the oracle's.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C0_CAM = dict(pos=(0.5, 1.0, 1.25), look_at=(0.5, 0.1, 0.25), up=(0, 1, 0), fov=45.0)
+BIG_CAM = dict(pos=(0.5, 0.5, 2.0), look_at=(0.5, 0.5, 0.5), up=(0, 1, 0), fov=45.0)
+
+
+@pytest.fixture(scope="module")
+def c0(bvh, orc):
+    v, f = bvh.scene_heightfield(256, 128, 1, 0.2)
+    mesh = bvh.DeviceMesh(v, f)
+    tree = bvh.lbvh_build(mesh)
+    nodes = tree.download()
+    assert nodes.tobytes() == orc.lbvh(v, f)[2].tobytes()  # trace only a tree known to be valid
+    return v, f, mesh, tree, nodes
+
+
+@pytest.fixture(scope="module")
+def spheres(bvh, orc):
+    v, f = bvh.scene_spheres_soup(256, 40000, 5)
+    mesh = bvh.DeviceMesh(v, f)
+    tree = bvh.lbvh_build(mesh)
+    nodes = tree.download()
+    assert nodes.tobytes() == orc.lbvh(v, f)[2].tobytes()
+    return v, f, mesh, tree, nodes
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 5), (640, 360)])
+def test_generate_rays_bit_exact(bvh, orc, w, h):
+    got = bvh.generate_rays(w, h, **C0_CAM)
+    expect = orc.gen_rays(w, h, C0_CAM["pos"], C0_CAM["look_at"], C0_CAM["up"], C0_CAM["fov"])
+    assert got.tobytes() == expect.tobytes()
+
+
+def test_trace_c0_bit_exact(bvh, orc, c0):
+    v, f, mesh, tree, nodes = c0
+    rays = bvh.generate_rays(512, 512, **C0_CAM)
+    pix, hits = bvh.trace_rays(tree, mesh, rays)
+    opix, ohits = orc.trace(rays, nodes, 0, v, f)
+    assert (hits["triId"] != 0xFFFFFFFF).mean() > 0.2
+    assert np.array_equal(hits["triId"], ohits["triId"])
+    assert np.array_equal(hits["t"], ohits["t"])
+    assert pix.tobytes() == opix.tobytes()
+
+
+def test_trace_image_command(bvh, orc, c0):
+    """generateRays + tracerImage + copyImageToHost (the CUDATracer command)."""
+    v, f, mesh, tree, nodes = c0
+    img = bvh.trace_image(tree, mesh, 320, 200, **C0_CAM)
+    rays = orc.gen_rays(320, 200, C0_CAM["pos"], C0_CAM["look_at"], C0_CAM["up"], C0_CAM["fov"])
+    opix, _ = orc.trace(rays, nodes, 0, v, f)
+    assert img.tobytes() == opix.tobytes()
+
+
+def test_trace_spheres_bit_exact(bvh, orc, spheres):
+    v, f, mesh, tree, nodes = spheres
+    rays = bvh.generate_rays(384, 216, **BIG_CAM)
+    pix, hits = bvh.trace_rays(tree, mesh, rays)
+    opix, ohits = orc.trace(rays, nodes, 0, v, f)
+    assert np.array_equal(hits, ohits) and pix.tobytes() == opix.tobytes()
+
+
+def test_bounce_rays(bvh, orc, spheres):
+    v, f, mesh, tree, nodes = spheres
+    rays = bvh.generate_rays(256, 144, **BIG_CAM)
+    _, hits = bvh.trace_rays(tree, mesh, rays)
+    brays, skip = bvh.bounce_rays(mesh, rays, hits, seed=7)
+    hit = hits["triId"] != 0xFFFFFFFF
+    assert hit.sum() > 1000
+    d = brays["dir"][hit].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    tv = v[f[hits["triId"][hit]]].astype(np.float64)
+    n = np.cross(tv[:, 1] - tv[:, 0], tv[:, 2] - tv[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    n *= -np.sign((n * rays["dir"][hit]).sum(1))[:, None]
+    cos = (n * d).sum(1)
+    assert np.all(cos > -1e-4)
+    assert cos.mean() == pytest.approx(2.0 / 3.0, abs=0.02)  # cosine-weighted: E[cos] = 2/3
+    assert np.array_equal(skip[hit], hits["triId"][hit]) and np.all(skip[~hit] == 0xFFFFFFFF)
+    pix, bh = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
+    opix, obh = orc.trace(brays, nodes, 0, v, f, mode=1, tmin=0.0, skip=skip)
+    assert np.array_equal(bh, obh) and pix.tobytes() == opix.tobytes()
