@@ -1,0 +1,178 @@
+"""The JSON command host (bvhtest_amd/bin/bvhtest): config handling, error
+paths and file formats on the CPU; the full GPU pipeline under -m gpu.
+
+Follows src/main.cpp:97-244 and lib/base/Command.cpp:35-69 (unknown commands
+are skipped with a warning, a JSON error ends the run with status 2, the
+first failing command ends the current input).
+"""
+import json
+import os
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "bvhtest_amd", "bin", "bvhtest")
+
+
+def cli(*args, cwd=None):
+    return subprocess.run([BIN, *args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                          timeout=300)
+
+
+def write_cfg(tmp_path, commands, inputs=("scene",), verbose=True):
+    cfg = {"baseConfig": {"name": "t", "verbose": verbose, "basePath": str(tmp_path), "input": list(inputs)},
+           "commands": commands}
+    p = tmp_path / "cfg.json"
+    p.write_text(json.dumps(cfg))
+    return p
+
+
+def test_list_and_generate():
+    r = cli("list")
+    assert r.returncode == 0
+    for name in ("load", "LBVH", "bittner13GPU", "CUDATracer", "copyToGPU", "copyToHost", "validate", "writeData"):
+        assert name in r.stdout
+    r = cli("generate", "-", "LBVH", "bittner13GPU", "CUDATracer", "copyToHost", "writeData")
+    cfg = json.loads(r.stdout)
+    opts = {c["cmd"]: c.get("options", {}) for c in cfg["commands"]}
+    # defaults of lib/builder/Bittner13GPU.hpp:28-37, OptimizerBase.cpp:28-33
+    assert opts["bittner13GPU"]["maxNumStepps"] == 500 and opts["bittner13GPU"]["numChunks"] == 32
+    assert opts["bittner13GPU"]["CUDABlockSize"] == 128 and opts["bittner13GPU"]["batchPercent"] == 1.0
+    assert opts["bittner13GPU"]["sort"] is True and opts["bittner13GPU"]["base_calcSAH"] is False
+    assert opts["copyToHost"]["fixLevels"] is True
+    assert opts["writeData"]["outFile"] == "%d/%i/rData_%n.json"
+    assert cli("generate", "-", "noSuchCommand").returncode == 2
+
+
+def test_json_error_and_unknown_command(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text("{ not json")
+    assert cli("run", str(p)).returncode == 2
+    p = write_cfg(tmp_path, [{"cmd": "noSuchCommand"}, {"cmd": "status"}], inputs=["x"])
+    r = cli("run", str(p))
+    assert r.returncode == 0 and "not found" in r.stderr and "Status:" in r.stdout
+    # the config is rewritten with the unknown command dropped
+    assert [c["cmd"] for c in json.loads(p.read_text())["commands"]] == ["status"]
+
+
+def test_missing_mesh_stops_the_input(tmp_path):
+    p = write_cfg(tmp_path, [{"cmd": "load"}, {"cmd": "status"}], inputs=["nothere"])
+    r = cli("run", str(p))
+    assert r.returncode == 0
+    assert "does not exist" in r.stderr and "Status:" not in r.stdout
+
+
+def test_generate_export_load_roundtrip(tmp_path):
+    p = write_cfg(tmp_path, [{"cmd": "genScene", "options": {"type": "heightfield", "gx": 16, "gz": 8}},
+                             {"cmd": "export", "options": {"outDir": "mesh"}}], inputs=["x"])
+    assert cli("run", str(p)).returncode == 0
+    ctl = json.loads((tmp_path / "mesh" / "data.json").read_text())
+    assert ctl["version"] == 3 and ctl["mesh"]["faces"] == 256 and ctl["mesh"]["vert"] == 17 * 9
+    p = write_cfg(tmp_path, [{"cmd": "load"}, {"cmd": "status"}], inputs=["mesh"])
+    r = cli("run", str(p))
+    assert "Faces:    256" in r.stdout
+    # unsupported format version
+    ctl["version"] = 7
+    (tmp_path / "mesh" / "data.json").write_text(json.dumps(ctl))
+    r = cli("run", str(p))
+    assert "version" in r.stderr and "Status:" not in r.stdout
+
+
+def test_bvh_import_checks_adler32(tmp_path, orc):
+    """BVHImport (format 9): raw nodes + Adler-32 (RFC 1950, here checked
+    against zlib.adler32) of the raw bytes."""
+    rng = np.random.default_rng(0)
+    n = 500
+    c = rng.random((n, 3), dtype=np.float32)
+    v = (c[:, None, :] + rng.random((n, 3, 3), dtype=np.float32) * 0.01).reshape(-1, 3).astype(np.float32)
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    _, _, nodes, _ = orc.lbvh(v, f)
+    ml = orc.fix_levels(nodes, 0)
+    d = tmp_path / "scene"
+    d.mkdir()
+    # mesh for the validate command
+    (d / "data.json").write_text(json.dumps({"version": 3, "bin": "data.bin",
+                                             "mesh": {"vert": len(v), "normals": 0, "faces": n}}))
+    (d / "data.bin").write_bytes(v.tobytes() + f.tobytes())
+    raw = nodes.tobytes()
+    (d / "tree_bvh.bin").write_bytes(raw)
+    ctl = {"version": 9, "bin": "tree_bvh.bin", "BVHSize": len(nodes), "treeHeight": ml,
+           "rawChecksum": zlib.adler32(raw)}
+    (d / "tree_bvh.json").write_text(json.dumps(ctl))
+    cmds = [{"cmd": "load"}, {"cmd": "BVHImport", "options": {"name": "tree"}}, {"cmd": "validate"},
+            {"cmd": "BVHExport", "options": {"name": "again"}}, {"cmd": "status"}]
+    p = write_cfg(tmp_path, cmds)
+    r = cli("run", str(p))
+    assert r.returncode == 0 and "error" not in r.stderr, r.stderr
+    assert (d / "again_bvh.bin").read_bytes() == raw
+    assert json.loads((d / "again_bvh.json").read_text())["rawChecksum"] == zlib.adler32(raw)
+    # a flipped byte is detected
+    bad = bytearray(raw)
+    bad[100] ^= 1
+    (d / "tree_bvh.bin").write_bytes(bytes(bad))
+    r = cli("run", str(p))
+    assert "corrupt" in r.stderr and "Status:" not in r.stdout
+
+
+def test_validate_reports_corruption(tmp_path, orc):
+    rng = np.random.default_rng(1)
+    n = 300
+    v = rng.random((3 * n, 3), dtype=np.float32)
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    _, _, nodes, _ = orc.lbvh(v, f)
+    ml = orc.fix_levels(nodes, 0)
+    nodes["bbox"][3][0] -= 1.0
+    d = tmp_path / "scene"
+    d.mkdir()
+    (d / "data.json").write_text(json.dumps({"version": 3, "bin": "data.bin",
+                                             "mesh": {"vert": len(v), "normals": 0, "faces": n}}))
+    (d / "data.bin").write_bytes(v.tobytes() + f.tobytes())
+    raw = nodes.tobytes()
+    (d / "t_bvh.bin").write_bytes(raw)
+    (d / "t_bvh.json").write_text(json.dumps({"version": 9, "bin": "t_bvh.bin", "BVHSize": len(nodes),
+                                              "treeHeight": ml, "rawChecksum": zlib.adler32(raw)}))
+    p = write_cfg(tmp_path, [{"cmd": "load"}, {"cmd": "BVHImport", "options": {"name": "t"}}, {"cmd": "validate"},
+                             {"cmd": "status"}])
+    r = cli("run", str(p))
+    # validate returns WARNING: reported, and the next command still runs
+    assert "invalid bounding boxes (strict)" in r.stderr and "Status:" in r.stdout
+
+
+@pytest.mark.gpu
+def test_full_gpu_pipeline(tmp_path, orc, bvh):
+    d = tmp_path / "scene"
+    d.mkdir()
+    cams = {"version": 1, "cameras": [{"type": "PERSPECTIVE", "cam": {
+        "camera": {"pos": {"x": 0.5, "y": 1.0, "z": 1.25}, "lookAt": {"x": 0.5, "y": 0.1, "z": 0.25},
+                   "up": {"x": 0, "y": 1, "z": 0}, "fov": 45.0},
+        "resolution": {"width": 256, "height": 256}}}]}
+    (d / "cameras.json").write_text(json.dumps(cams))
+    base = [{"cmd": "genScene", "options": {"type": "heightfield", "gx": 64, "gz": 32, "seed": 1}},
+            {"cmd": "camImport"}, {"cmd": "copyToGPU"}, {"cmd": "LBVH"}]
+    tail = [{"cmd": "CUDATracer"}, {"cmd": "copyToHost"}, {"cmd": "validate"}, {"cmd": "status"},
+            {"cmd": "writeData"}, {"cmd": "BVHExport", "options": {"name": "out"}}]
+    # plain build: the exported tree equals the oracle's after fixLevels
+    p = write_cfg(tmp_path, base + tail)
+    r = cli("run", str(p))
+    assert r.returncode == 0 and "[error]" not in r.stderr, r.stderr
+    v, f = bvh.scene_heightfield(64, 32, 1, 0.2)
+    _, _, nodes, _ = orc.lbvh(v, f)
+    orc.fix_levels(nodes, 0)
+    assert (d / "out_bvh.bin").read_bytes() == nodes.tobytes()
+    # with the optimizer: valid tree, per-step data written
+    opt = [{"cmd": "bittner13GPU", "options": {"maxNumStepps": 5, "batchPercent": 5.0, "numChunks": 4,
+                                               "base_calcSAH": True, "base_doSync": True}}]
+    p = write_cfg(tmp_path, base + opt + tail)
+    r = cli("run", str(p))
+    assert r.returncode == 0 and "[error]" not in r.stderr, r.stderr
+    data = json.loads((d / "rData_t.json").read_text())
+    assert len(data["optStepps"]) == 6 and data["numFaces"] == len(f)
+    sah = [s["sah"] for s in data["optStepps"]]
+    assert sah[-1] < sah[0]
+    out = np.frombuffer((d / "out_bvh.bin").read_bytes(), dtype=orc.NODE_DTYPE)
+    ml = orc.fix_levels(out.copy(), 0)
+    total, err = orc.validate(out, 0, ml, v, f)
+    assert total == 0, err
