@@ -406,10 +406,10 @@ class Bittner13GPU:
         return sahs
 
     def stats(self):
-        out = (ctypes.c_uint32 * 5)()
-        lib().bvht_bn13_stats(ctypes.byref(self.mem), out, 5)
+        out = (ctypes.c_uint32 * 6)()
+        lib().bvht_bn13_stats(ctypes.byref(self.mem), out, 6)
         return dict(conflicts=out[0], failed=out[1], contained=out[2], overflows=out[3], applied=out[4],
-                    skipped=lib().calcNumSkipped(ctypes.byref(self.mem)))
+                    unresolved=out[5], skipped=lib().calcNumSkipped(ctypes.byref(self.mem)))
 
     def free(self):
         if self.mem is not None and self.mem.result:
@@ -425,3 +425,68 @@ class Bittner13GPU:
 
 def calc_sah(tree, c_inner=1.0, c_leaf=1.0):
     return lib().bvht_calcSAHWeighted(ctypes.byref(tree.ptr), c_inner, c_leaf)
+
+
+# ---------------------------------------------------------------------------
+# batched (segmented) builds
+# ---------------------------------------------------------------------------
+
+def scene_batch(num_meshes=512, seed=1000, min_tris=1000, max_tris=500000):
+    """Concatenated meshes; returns (vert, faces, face_counts, vert_counts)."""
+    from ._abi import LBVHBatchWorkingMemory  # noqa: F401
+    L = lib()
+    fc = np.empty(num_meshes, np.uint32)
+    vc = np.empty(num_meshes, np.uint32)
+    tv, tf = ctypes.c_uint64(), ctypes.c_uint64()
+    L.bvht_sceneBatch(num_meshes, seed, min_tris, max_tris, _ptr(fc), _ptr(vc), None, None, ctypes.byref(tv),
+                      ctypes.byref(tf))
+    v = np.empty((tv.value, 3), np.float32)
+    f = np.empty((tf.value, 3), np.uint32)
+    L.bvht_sceneBatch(num_meshes, seed, min_tris, max_tris, _ptr(fc), _ptr(vc), _ptr(v), _ptr(f), ctypes.byref(tv),
+                      ctypes.byref(tf))
+    return v, f, fc, vc
+
+
+class BatchBuilder:
+    """Segmented LBVH build of concatenated meshes (one launch sequence, optional hipGraph)."""
+
+    def __init__(self, vert, faces, face_counts, vert_counts, tile_log2=10):
+        L = lib()
+        fc = np.ascontiguousarray(face_counts, np.uint32)
+        vc = np.ascontiguousarray(vert_counts, np.uint32)
+        self.mem = L.bvht_LBVH_allocateBatch(len(fc), fc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                             vc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), tile_log2)
+        if not self.mem.lRes:
+            _fail("bvht_LBVH_allocateBatch")
+        self.face_counts, self.vert_counts = fc, vc
+        self.vb = DeviceBuffer(vert.nbytes).upload(np.ascontiguousarray(vert, np.float32))
+        self.fb = DeviceBuffer(faces.nbytes).upload(np.ascontiguousarray(faces, np.uint32))
+        self.nb = DeviceBuffer(self.mem.totalNodes * NODE_DTYPE.itemsize)
+
+    def build(self, use_graph=False):
+        if not lib().bvht_LBVH_buildBatch(ctypes.byref(self.mem), self.vb.ptr, self.fb.ptr, self.nb.ptr, use_graph):
+            _fail("bvht_LBVH_buildBatch")
+
+    def autotune(self):
+        ms = (ctypes.c_float * 4)()
+        best = lib().bvht_LBVH_autotuneBatch(ctypes.byref(self.mem), self.vb.ptr, self.fb.ptr, self.nb.ptr, ms)
+        return best, list(ms)
+
+    def offsets(self, i):
+        fo, vo, no = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        lib().bvht_LBVH_batchOffsets(ctypes.byref(self.mem), i, ctypes.byref(fo), ctypes.byref(vo), ctypes.byref(no))
+        return fo.value, vo.value, no.value
+
+    def nodes(self):
+        lib().LBVH_doCUDASyc()
+        return self.nb.download(NODE_DTYPE, self.mem.totalNodes)
+
+    def free(self):
+        if self.mem.lRes:
+            lib().bvht_LBVH_freeBatch(ctypes.byref(self.mem))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -46,6 +46,11 @@ class LBVHWorkingMemory(ctypes.Structure):
                 ("atomicLocks", c_vp), ("numLocks", c_u32), ("internal", c_vp)]
 
 
+class LBVHBatchWorkingMemory(ctypes.Structure):
+    _fields_ = [("lRes", c_bool), ("numMeshes", c_u32), ("totalFaces", c_u32), ("totalVert", c_u32),
+                ("totalNodes", c_u32), ("tileLog2", c_u32), ("internal", c_vp)]
+
+
 class AlgoCFG(ctypes.Structure):
     _fields_ = [("numChunks", c_u32), ("chunkSize", c_u32), ("blockSize", c_u32),
                 ("offsetAccess", c_bool), ("altFindNode", c_bool), ("altFixTree", c_bool),
@@ -108,6 +113,12 @@ SIGNATURES = {
     "bvht_LBVH_setTiming": (None, [c_bool]),
     "bvht_LBVH_stageTimes": (ctypes.c_int, [P(c_f), ctypes.c_int]),
     "bvht_LBVH_stageName": (ctypes.c_char_p, [ctypes.c_int]),
+    "bvht_LBVH_autotune": (ctypes.c_int, [P(LBVHWorkingMemory), P(MeshRaw), P(CUDAMemoryBVHPointer), P(c_f)]),
+    "bvht_LBVH_allocateBatch": (LBVHBatchWorkingMemory, [c_u32, P(c_u32), P(c_u32), c_u32]),
+    "bvht_LBVH_buildBatch": (c_bool, [P(LBVHBatchWorkingMemory), c_vp, c_vp, c_vp, c_bool]),
+    "bvht_LBVH_freeBatch": (None, [P(LBVHBatchWorkingMemory)]),
+    "bvht_LBVH_batchOffsets": (c_bool, [P(LBVHBatchWorkingMemory), c_u32, P(c_u32), P(c_u32), P(c_u32)]),
+    "bvht_LBVH_autotuneBatch": (ctypes.c_int, [P(LBVHBatchWorkingMemory), c_vp, c_vp, c_vp, P(c_f)]),
     "allocateMemory": (GPUWorkingMemory, [P(CUDAMemoryBVHPointer), c_u32, c_u32]),
     "freeMemory": (None, [P(GPUWorkingMemory)]),
     "initData": (None, [P(GPUWorkingMemory), P(CUDAMemoryBVHPointer), c_u32]),
@@ -134,6 +145,8 @@ SIGNATURES = {
     "bvht_sceneHeightfield": (None, [c_u32, c_u32, c_u32, c_f, c_vp, c_vp, P(c_u32), P(c_u32)]),
     "bvht_sceneSpheresSoup": (None, [c_u32, c_u32, c_u32, c_vp, c_vp, P(c_u32), P(c_u32)]),
     "bvht_sceneSoup": (None, [c_u32, c_u32, c_f, c_vp, c_vp, P(c_u32), P(c_u32)]),
+    "bvht_sceneBatch": (None, [c_u32, c_u32, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp, P(ctypes.c_uint64),
+                               P(ctypes.c_uint64)]),
 }
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")

@@ -10,9 +10,10 @@
 //               reinsertions, each found by a wave-parallel exact branch and
 //               bound over the patched view (the reference's findNode1 keeps
 //               a 4096-entry heap per thread in local memory); the patch
-//               locks its nodes with atomicMin(rank);
-//     k_resolve a patch wins iff it holds every lock (lowest rank wins:
-//               deterministic, unlike the reference's first-come CAS);
+//               reserves its nodes (round 1);
+//     k_commitFirst / k_resolveTail  deterministic reservations: exactly the
+//               patches a rank-ordered first-come locking applies (the
+//               reference's kCheckConflicts, timing-dependent there);
 //     k_contain a winner whose insertion point lies in a subtree another
 //               winner moves is dropped (two such patches would form a cycle;
 //               the reference has no guard);
@@ -42,7 +43,18 @@ constexpr int kStackCap = 1024;  // per-search frontier (findNode1 heap: 4096)
 constexpr int kStackCapAlt = 16; // altFindNode: bounded frontier (findNode2 keeps 16 slots)
 
 // Stats words (device)
-enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_WORDS = 8 };
+enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_UNRESOLVED, ST_WORDS = 8 };
+
+// Patch states and node locks.  A lock holds key(round, rank): the upper word
+// is ~round, so later rounds and applied patches (round kRoundWin, upper word
+// 0) compare smaller.
+constexpr uint64_t kLockFree = ~0ull;
+constexpr uint32_t kRoundWin = 0xFFFFFFFFu;
+enum : uint32_t { REC_NONE = 0, REC_PENDING = 1, REC_LOST = 2, REC_CONTAINED = 3, REC_WON = 4 };
+
+__device__ __forceinline__ uint64_t lockKey(uint32_t round, uint32_t k) {
+  return ((uint64_t)(0xFFFFFFFFu - round) << 32) | k;
+}
 
 struct PatchRec {
   uint32_t node[kNNode];
@@ -341,7 +353,7 @@ __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ 
 __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ nodes, uint32_t root,
                                                const uint32_t *__restrict__ sortedIds, uint32_t chunk,
                                                uint32_t numChunks, uint32_t K, uint32_t offsetAccess, int cap,
-                                               PatchRec *__restrict__ recs, uint32_t *__restrict__ locks,
+                                               PatchRec *__restrict__ recs, uint64_t *locks,
                                                uint32_t *__restrict__ stats) {
   __shared__ Patch P;
   __shared__ uint32_t s_node[kStackCap];
@@ -382,7 +394,7 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
   }
   if (!s_ok) {
     if (lane == 0) {
-      rec->ok = 0;
+      rec->ok = REC_NONE;
       atomicAdd(stats + ST_FAILED, 1u);
     }
     return;
@@ -394,7 +406,8 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
     rec->left[lane] = P.left[lane];
     rec->right[lane] = P.right[lane];
     rec->isLeft[lane] = P.isLeft[lane];
-    if (n != NONE) atomicMin(locks + n, k);
+    // round-1 reservation of the deterministic conflict resolution
+    if (n != NONE) atomicMin(reinterpret_cast<unsigned long long *>(locks + n), (unsigned long long)lockKey(1, k));
   }
   if (lane == 0) {
     rec->moved[0] = s_ins[0];
@@ -404,31 +417,107 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
     rec->fix[0] = s_gp;
     rec->fix[1] = s_unused[0];
     rec->fix[2] = s_unused[1];
-    rec->ok = 1;
+    rec->ok = REC_PENDING;
   }
 }
 
-// A patch wins iff it holds the lock (lowest rank) on every node it touches.
-__global__ __launch_bounds__(256) void k_resolve(PatchRec *__restrict__ recs, uint32_t K,
-                                                 const uint32_t *__restrict__ locks, uint32_t *__restrict__ marks,
-                                                 uint32_t *__restrict__ skipped, uint32_t *__restrict__ stats) {
+// ---- conflict resolution: deterministic reservations ---------------------
+//
+// kCheckConflicts (Bittner13CUDA.cu:567-609) lets patch k lock its nodes
+// first come, first served; serialised in rank order this is the greedy rule
+// "k is applied iff no node of k belongs to an applied patch of lower rank".
+// That rule is computed here exactly and independently of timing: in round r
+// every undecided patch reserves its nodes with atomicMin(key(r, k)); a patch
+// that holds every reservation is applied (its nodes get key(WIN, k), smaller
+// than any reservation), a patch that meets an applied patch's node loses,
+// the rest retry in round r + 1.  The undecided patch of lowest rank always
+// decides, so a round never leaves the set unchanged.
+__device__ __forceinline__ void reserve(const PatchRec *r, uint32_t k, uint32_t round, uint64_t *locks) {
+  const uint64_t key = lockKey(round, k);
+  for (int i = 0; i < kNNode; ++i) {
+    const uint32_t n = r->node[i];
+    if (n != NONE) atomicMin(reinterpret_cast<unsigned long long *>(locks + n), (unsigned long long)key);
+  }
+}
+
+// Returns REC_WON, REC_LOST or REC_PENDING and records a decision.
+__device__ uint32_t decide(PatchRec *r, uint32_t k, uint32_t round, uint64_t *locks, uint32_t *marks,
+                           uint32_t *skipped, uint32_t *stats) {
+  const uint64_t mine = lockKey(round, k);
+  bool holdAll = true, lost = false;
+  for (int i = 0; i < kNNode; ++i) {
+    const uint32_t n = r->node[i];
+    if (n == NONE) continue;
+    const uint64_t v = __hip_atomic_load(locks + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((v >> 32) == 0) lost = true;  // node of an applied patch
+    if (v != mine) holdAll = false;
+  }
+  if (lost) {
+    r->ok = REC_LOST;
+    skipped[k] += 1;
+    atomicAdd(stats + ST_CONFLICT, 1u);
+    return REC_LOST;
+  }
+  if (!holdAll) return REC_PENDING;
+  const uint64_t win = lockKey(kRoundWin, k);
+  for (int i = 0; i < kNNode; ++i) {
+    const uint32_t n = r->node[i];
+    if (n != NONE) atomicMin(reinterpret_cast<unsigned long long *>(locks + n), (unsigned long long)win);
+  }
+  r->ok = REC_WON;
+  marks[r->moved[0]] = k;  // moved roots are patch nodes: one applied patch each
+  marks[r->moved[1]] = k;
+  return REC_WON;
+}
+
+// Round 1 over every patch (the reservations were made by k_search).
+__global__ __launch_bounds__(256) void k_commitFirst(PatchRec *__restrict__ recs, uint32_t K, uint64_t *locks,
+                                                     uint32_t *__restrict__ marks, uint32_t *__restrict__ skipped,
+                                                     uint32_t *__restrict__ stats, uint32_t *__restrict__ pending,
+                                                     uint32_t *__restrict__ numPending) {
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= K) return;
   PatchRec *r = recs + k;
-  if (!r->ok) return;
-  bool win = true;
-  for (int i = 0; i < kNNode; ++i) {
-    const uint32_t n = r->node[i];
-    if (n != NONE && locks[n] != k) win = false;
+  if (r->ok != REC_PENDING) return;
+  if (decide(r, k, 1, locks, marks, skipped, stats) == REC_PENDING) pending[atomicAdd(numPending, 1u)] = k;
+}
+
+// Rounds 2.. over the (few) undecided patches, in one workgroup.  At most
+// `num + 1` rounds are needed; patches still undecided after that bound
+// (never, unless memory ordering is broken) are dropped and counted.
+constexpr int kTailBlock = 1024;
+__global__ __launch_bounds__(kTailBlock) void k_resolveTail(PatchRec *__restrict__ recs, uint64_t *locks,
+                                                           uint32_t *__restrict__ marks,
+                                                           uint32_t *__restrict__ skipped, uint32_t *__restrict__ stats,
+                                                           uint32_t *listA, uint32_t *listB,
+                                                           const uint32_t *__restrict__ numPending) {
+  __shared__ uint32_t s_cnt;
+  uint32_t *cur = listA, *nxt = listB;
+  uint32_t num = *numPending;
+  const uint32_t maxRound = num + 2;
+  uint32_t round = 2;
+  for (; num > 0 && round <= maxRound; ++round) {
+    for (uint32_t i = threadIdx.x; i < num; i += kTailBlock) reserve(recs + cur[i], cur[i], round, locks);
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < num; i += kTailBlock) {
+      const uint32_t k = cur[i];
+      if (decide(recs + k, k, round, locks, marks, skipped, stats) == REC_PENDING) nxt[atomicAdd(&s_cnt, 1u)] = k;
+    }
+    __syncthreads();
+    num = s_cnt;
+    uint32_t *t = cur;
+    cur = nxt;
+    nxt = t;
+    __syncthreads();
   }
-  if (!win) {
-    r->ok = 2;  // lost a lock
-    skipped[k] += 1;
+  for (uint32_t i = threadIdx.x; i < num; i += kTailBlock) {
+    PatchRec *r = recs + cur[i];
+    r->ok = REC_LOST;
+    skipped[cur[i]] += 1;
     atomicAdd(stats + ST_CONFLICT, 1u);
-    return;
+    atomicAdd(stats + ST_UNRESOLVED, 1u);
   }
-  atomicMin(marks + r->moved[0], k);
-  atomicMin(marks + r->moved[1], k);
 }
 
 // Drops a winner whose insertion point lies below a node another winner moves.
@@ -439,7 +528,7 @@ __global__ __launch_bounds__(256) void k_contain(PatchRec *__restrict__ recs, ui
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= K) return;
   PatchRec *r = recs + k;
-  if (r->ok != 1) return;
+  if (r->ok != REC_WON) return;
   bool bad = false;
   for (int w = 0; w < 2 && !bad; ++w) {
     uint32_t x = r->best[w];
@@ -454,7 +543,7 @@ __global__ __launch_bounds__(256) void k_contain(PatchRec *__restrict__ recs, ui
     }
   }
   if (bad) {
-    r->ok = 3;
+    r->ok = REC_CONTAINED;
     skipped[k] += 1;
     atomicAdd(stats + ST_CONTAIN, 1u);
   }
@@ -462,19 +551,21 @@ __global__ __launch_bounds__(256) void k_contain(PatchRec *__restrict__ recs, ui
 
 // Winners write their topology; every patch releases its locks and marks.
 __global__ __launch_bounds__(256) void k_apply(PatchRec *__restrict__ recs, uint32_t K, bvht_BVHNode *__restrict__ nodes,
-                                               uint32_t *__restrict__ locks, uint32_t *__restrict__ marks,
-                                               uint32_t *__restrict__ fixStarts, uint32_t *__restrict__ stats) {
+                                               uint64_t *__restrict__ locks, uint32_t *__restrict__ marks,
+                                               uint32_t *__restrict__ fixStarts, uint32_t *__restrict__ numPending,
+                                               uint32_t *__restrict__ stats) {
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= K) return;
+  if (k == 0) *numPending = 0;
   PatchRec *r = recs + k;
   const uint32_t ok = r->ok;
-  const bool win = ok == 1;
+  const bool win = ok == REC_WON;
   for (int i = 0; i < 3; ++i) fixStarts[3 * k + i] = win ? r->fix[i] : NONE;
-  if (ok == 0) return;
+  if (ok == REC_NONE) return;
   for (int i = 0; i < kNNode; ++i) {
     const uint32_t n = r->node[i];
     if (n == NONE) continue;
-    locks[n] = NONE;
+    locks[n] = kLockFree;
     if (win) {  // MiniPatch::applyOne -- BVHPatch.hpp:72-80
       nodes[n].parent = r->parent[i];
       nodes[n].left = r->left[i];
@@ -484,7 +575,7 @@ __global__ __launch_bounds__(256) void k_apply(PatchRec *__restrict__ recs, uint
   }
   marks[r->moved[0]] = NONE;
   marks[r->moved[1]] = NONE;
-  r->ok = 0;
+  r->ok = REC_NONE;
   if (win) atomicAdd(stats + ST_APPLIED, 1u);
 }
 
@@ -623,7 +714,9 @@ struct Internal {
   SortPlan plan;
   uint32_t *sorted = nullptr;  // -> vals[result]
   PatchRec *recs = nullptr;
-  uint32_t *locks = nullptr, *marks = nullptr, *cnt = nullptr, *refitCnt = nullptr;
+  uint64_t *locks = nullptr;
+  uint32_t *marks = nullptr, *cnt = nullptr, *refitCnt = nullptr;
+  uint32_t *pending[2] = {nullptr, nullptr}, *numPending = nullptr;
   uint32_t *fixStarts = nullptr;
   uint32_t *skipped = nullptr;
   uint32_t *stats = nullptr;
@@ -713,6 +806,8 @@ bvht_GPUWorkingMemory allocateMemory(bvht_CUDAMemoryBVHPointer *bvh, uint32_t ba
             devAlloc(&in->vals[0], in->numInner, "bn13 vals") && devAlloc(&in->vals[1], in->numInner, "bn13 vals") &&
             devMalloc(reinterpret_cast<void **>(&in->sortScratch), sortScratchBytes(in->plan), "bn13 sort") &&
             devAlloc(&in->recs, batchSize, "bn13 patches") && devAlloc(&in->locks, in->N, "bn13 locks") &&
+            devAlloc(&in->pending[0], batchSize, "bn13 pending") && devAlloc(&in->pending[1], batchSize, "bn13 pending") &&
+            devAlloc(&in->numPending, 1, "bn13 pending count") &&
             devAlloc(&in->marks, in->N, "bn13 marks") && devAlloc(&in->cnt, in->N, "bn13 refit counters") &&
             devAlloc(&in->refitCnt, in->N, "bn13 refit counters") &&
             devAlloc(&in->fixStarts, 3 * (size_t)batchSize, "bn13 fix starts") &&
@@ -738,6 +833,9 @@ void freeMemory(bvht_GPUWorkingMemory *data) {
     devFree(in->sortScratch);
     devFree(in->recs);
     devFree(in->locks);
+    devFree(in->pending[0]);
+    devFree(in->pending[1]);
+    devFree(in->numPending);
     devFree(in->marks);
     devFree(in->cnt);
     devFree(in->refitCnt);
@@ -789,7 +887,8 @@ void initData(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint3
     return;
   k_selWrite<<<selBlocks, kSelBlock, 0, stream()>>>(bvh->nodes, in->N, in->selCounts, in->inner, in->leaves);
   if (!checkLaunch("k_selWrite")) return;
-  fillU32(in->locks, NONE, in->N);
+  fillU32(reinterpret_cast<uint32_t *>(in->locks), NONE, 2ull * in->N);  // kLockFree
+  fillU32(in->numPending, 0u, 1);
   fillU32(in->marks, NONE, in->N);
   fillU32(in->cnt, 0u, in->N);
   fillU32(in->refitCnt, 0u, in->N);
@@ -860,12 +959,17 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
   k_search<<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->sorted, chunk, cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u,
                                    cap, in->recs, in->locks, in->stats);
   if (!checkLaunch("k_search")) return;
-  k_resolve<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats);
-  if (!checkLaunch("k_resolve")) return;
+  k_commitFirst<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats,
+                                                     in->pending[0], in->numPending);
+  if (!checkLaunch("k_commitFirst")) return;
+  k_resolveTail<<<1, kTailBlock, 0, stream()>>>(in->recs, in->locks, in->marks, in->skipped, in->stats, in->pending[0],
+                                                in->pending[1], in->numPending);
+  if (!checkLaunch("k_resolveTail")) return;
   k_contain<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, bvh->nodes, in->root, in->N, in->marks, in->skipped,
                                                  in->stats);
   if (!checkLaunch("k_contain")) return;
-  k_apply<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, bvh->nodes, in->locks, in->marks, in->fixStarts, in->stats);
+  k_apply<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, bvh->nodes, in->locks, in->marks, in->fixStarts,
+                                               in->numPending, in->stats);
   if (!checkLaunch("k_apply")) return;
   const uint32_t num = 3 * K;
   k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
@@ -915,7 +1019,8 @@ uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data) {
 }
 
 // Counters of the optimizer (extension): conflicts, failed patches,
-// containment drops, frontier overflows, applied patches.
+// containment drops, frontier overflows, applied patches, patches left
+// undecided by the conflict resolution (always 0).
 int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
   Internal *in = inOf(data);
   if (!in || !out) return 0;
@@ -923,7 +1028,7 @@ int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
   if (!check(hipMemcpyAsync(h, in->stats, sizeof(h), hipMemcpyDeviceToHost, stream()), "bn13 stats") ||
       !check(hipStreamSynchronize(stream()), "bn13 stats sync"))
     return 0;
-  int n = maxWords < 5 ? maxWords : 5;
+  int n = maxWords < 6 ? maxWords : 6;
   for (int i = 0; i < n; ++i) out[i] = h[i];
   return n;
 }

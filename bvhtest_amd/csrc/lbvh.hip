@@ -16,6 +16,7 @@
 // The reference topology is a closed form of n (see types.hpp), so no key is
 // read to build it and no global atomic is needed for the refit.
 #include "lbvh.hpp"
+#include "lbvh_kernels.hpp"
 #include "radix_sort.hpp"
 #include "refit.hpp"
 #include "wave.hpp"
@@ -28,19 +29,8 @@ namespace bvht {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTileLog = 10;  // 1024 units per build workgroup
+constexpr int kTileLog = 10;  // default: 1024 units per build workgroup
 constexpr uint32_t kBoxBlocksMax = 1024;
-
-__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
-
-// Writes one full 48-byte node with three 16-byte stores.
-__device__ __forceinline__ void storeNode(bvht_BVHNode *dst, const bvht_AABB &b, uint32_t parent, uint32_t left,
-                                          uint32_t right, uint32_t isLeaf, uint32_t isLeft, uint32_t level, float sa) {
-  uint4 *d = reinterpret_cast<uint4 *>(dst);
-  d[0] = make_uint4(f2u(b.minMax[0].x), f2u(b.minMax[0].y), f2u(b.minMax[0].z), f2u(b.minMax[1].x));
-  d[1] = make_uint4(f2u(b.minMax[1].y), f2u(b.minMax[1].z), parent, left);
-  d[2] = make_uint4(right, isLeaf | (isLeft << 16), level, f2u(sa));
-}
 
 __device__ __forceinline__ bvht_vec3 ldVec3(const bvht_vec3 *p) { return *p; }
 
@@ -151,93 +141,6 @@ __global__ __launch_bounds__(kBlock) void k_morton(const bvht_vec3 *__restrict__
   for (int p = 0; p < PASSES; ++p) {
     uint32_t c = s_hist[p][threadIdx.x];
     if (c) atomicAdd(&hist[p * kSortRadix + threadIdx.x], c);
-  }
-}
-
-// Parent index and isLeft flag of the node covering [l, r].
-__device__ __forceinline__ void parentOf(uint32_t l, uint32_t r, uint32_t N1, uint32_t &parent, uint32_t &isLeft) {
-  if (l == 0 && r == N1) {
-    parent = 0;  // the root is its own parent
-    isLeft = 0;
-    return;
-  }
-  uint32_t pl, pr;
-  rangeParent(l, r, N1, pl, pr);
-  parent = rangeNodeIndex(pl, pr, N1);
-  isLeft = rangeIsLeft(l, r, N1) ? 1u : 0u;
-}
-
-// One level of the tile hierarchy.  Units are aligned runs of 2^unitLog leaf
-// positions; a workgroup owns 2^kTileLog consecutive units and writes every
-// node whose leaf range is a union of 2..2^kTileLog of its units.
-template <bool LEVEL0>
-__global__ __launch_bounds__(kBlock) void k_buildLevel(bvht_BVHNode *__restrict__ nodes,
-                                                       const uint32_t *__restrict__ sortedFaces,
-                                                       const bvht_Triangle *__restrict__ faces,
-                                                       const bvht_vec3 *__restrict__ vert, uint32_t n,
-                                                       uint32_t unitLog) {
-  constexpr uint32_t T = 1u << kTileLog;
-  __shared__ float s_box[6][T];
-  const uint32_t N1 = n - 1;
-  const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
-  const uint64_t u0 = (uint64_t)blockIdx.x << kTileLog;
-  const uint32_t cnt = (uint32_t)min((uint64_t)T, U - u0);
-
-  for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-    const uint64_t u = u0 + k;
-    bvht_AABB b;
-    if (LEVEL0) {
-      const uint32_t p = (uint32_t)u;
-      uint32_t f = sortedFaces[p];
-      if (f >= n) f = 0;  // never read outside the mesh, even from a corrupt permutation
-      const bvht_Triangle t = faces[f];
-      b = triangleBox(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
-      uint32_t parent, isLeft;
-      parentOf(p, p, N1, parent, isLeft);
-      // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
-      storeNode(nodes + N1 + p, b, parent, f, 1u, 1u, isLeft, 255u, surfaceArea(b));
-    } else {
-      const uint32_t l = (uint32_t)(u << unitLog);
-      const uint32_t r = (uint32_t)min((u + 1) << unitLog, (uint64_t)n) - 1;
-      const float *src = reinterpret_cast<const float *>(nodes + rangeNodeIndex(l, r, N1));
-      b.minMax[0] = {src[0], src[1], src[2]};
-      b.minMax[1] = {src[3], src[4], src[5]};
-    }
-    s_box[0][k] = b.minMax[0].x;
-    s_box[1][k] = b.minMax[0].y;
-    s_box[2][k] = b.minMax[0].z;
-    s_box[3][k] = b.minMax[1].x;
-    s_box[4][k] = b.minMax[1].y;
-    s_box[5][k] = b.minMax[1].z;
-  }
-  __syncthreads();
-
-  for (uint32_t j = 1; j <= kTileLog; ++j) {
-    const uint32_t half = 1u << (j - 1);
-    const uint32_t nb = (cnt + (1u << j) - 1) >> j;
-    for (uint32_t a = threadIdx.x; a < nb; a += kBlock) {
-      const uint32_t k0 = a << j, km = k0 + half;
-      if (km >= cnt) continue;  // right half empty: same range as the left half
-      bvht_AABB b;
-      b.minMax[0] = {mergeMin(s_box[0][k0], s_box[0][km]), mergeMin(s_box[1][k0], s_box[1][km]),
-                     mergeMin(s_box[2][k0], s_box[2][km])};
-      b.minMax[1] = {mergeMax(s_box[3][k0], s_box[3][km]), mergeMax(s_box[4][k0], s_box[4][km]),
-                     mergeMax(s_box[5][k0], s_box[5][km])};
-      s_box[0][k0] = b.minMax[0].x;
-      s_box[1][k0] = b.minMax[0].y;
-      s_box[2][k0] = b.minMax[0].z;
-      s_box[3][k0] = b.minMax[1].x;
-      s_box[4][k0] = b.minMax[1].y;
-      s_box[5][k0] = b.minMax[1].z;
-      const uint32_t l = (uint32_t)((u0 + k0) << unitLog);
-      const uint32_t m = (uint32_t)((u0 + km) << unitLog);
-      const uint32_t r = (uint32_t)min((u0 + k0 + (1u << j)) << unitLog, (uint64_t)n) - 1;
-      uint32_t parent, isLeft;
-      parentOf(l, r, N1, parent, isLeft);
-      storeNode(nodes + rangeNodeIndex(l, r, N1), b, parent, rangeNodeIndex(l, m - 1, N1), rangeNodeIndex(m, r, N1), 0u,
-                isLeft, 0u, surfaceArea(b));
-    }
-    __syncthreads();
   }
 }
 
@@ -411,25 +314,79 @@ bool runTree(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMe
     }
     return true;
   }
+  return launchTreeLevels((int)mem->opts.tileLog2, bvh->nodes, mem->sortedValues, mesh->faces, mesh->vert, n,
+                          timing().enabled ? [](bool l0) { timing().record(l0 ? "k_buildLevel0" : "k_buildLevelUp"); }
+                                           : nullptr);
+}
+
+}  // namespace
+
+template <int TLOG>
+static bool treeLevelsT(bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces, const bvht_vec3 *vert,
+                        uint32_t n, void (*afterLevel)(bool)) {
+  const SegDesc sd = {n, 0, 0, 0};
   uint32_t unitLog = 0;
   bool first = true;
   while (true) {
     const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
-    const uint32_t blocks = (uint32_t)((U + (1u << kTileLog) - 1) >> kTileLog);
+    const uint32_t blocks = (uint32_t)((U + (1u << TLOG) - 1) >> TLOG);
     if (first)
-      k_buildLevel<true><<<blocks, kBlock, 0, stream()>>>(bvh->nodes, mem->sortedValues, mesh->faces, mesh->vert, n, 0);
+      k_buildLevel<TLOG, true, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, sd, nullptr,
+                                                                              nullptr, 0);
     else
-      k_buildLevel<false><<<blocks, kBlock, 0, stream()>>>(bvh->nodes, nullptr, mesh->faces, mesh->vert, n, unitLog);
+      k_buildLevel<TLOG, false, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, nullptr, faces, vert, sd, nullptr,
+                                                                               nullptr, unitLog);
     if (!checkLaunch("k_buildLevel")) return false;
-    timing().record(first ? "k_buildLevel0" : "k_buildLevelUp");
+    if (afterLevel) afterLevel(first);
     first = false;
     if (blocks <= 1) break;
-    unitLog += kTileLog;
+    unitLog += TLOG;
   }
   return true;
 }
 
-}  // namespace
+bool launchTreeLevels(int tlog, bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
+                      const bvht_vec3 *vert, uint32_t n, void (*afterLevel)(bool)) {
+  switch (tlog) {
+    case 8: return treeLevelsT<8>(nodes, sorted, faces, vert, n, afterLevel);
+    case 9: return treeLevelsT<9>(nodes, sorted, faces, vert, n, afterLevel);
+    case 10: return treeLevelsT<10>(nodes, sorted, faces, vert, n, afterLevel);
+    case 11: return treeLevelsT<11>(nodes, sorted, faces, vert, n, afterLevel);
+  }
+  setError("launchTreeLevels: unsupported tile log2 %d", tlog);
+  return false;
+}
+
+template <int TLOG>
+static bool treeLevelsBatchT(bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
+                             const bvht_vec3 *vert, const SegDesc *segs, const uint2 *const *tiles,
+                             const uint32_t *counts, int levels) {
+  const SegDesc none = {0, 0, 0, 0};
+  for (int L = 0; L < levels; ++L) {
+    if (counts[L] == 0) continue;
+    if (L == 0)
+      k_buildLevel<TLOG, true, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, none, segs,
+                                                                                tiles[L], 0);
+    else
+      k_buildLevel<TLOG, false, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, nullptr, faces, vert, none, segs,
+                                                                                 tiles[L], (uint32_t)(L * TLOG));
+    if (!checkLaunch("k_buildLevel batch")) return false;
+  }
+  return true;
+}
+
+bool launchTreeLevelsBatch(int tlog, bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
+                           const bvht_vec3 *vert, const SegDesc *segs, const uint2 *const *tiles,
+                           const uint32_t *counts, int levels) {
+  switch (tlog) {
+    case 8: return treeLevelsBatchT<8>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
+    case 9: return treeLevelsBatchT<9>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
+    case 10: return treeLevelsBatchT<10>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
+    case 11: return treeLevelsBatchT<11>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
+  }
+  setError("launchTreeLevelsBatch: unsupported tile log2 %d", tlog);
+  return false;
+}
 
 }  // namespace bvht
 
@@ -486,7 +443,11 @@ bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *me
     setError("LBVH_allocateWorkingMemory: topology must be 0 or 1 (got %u)", m.opts.topology);
     return m;
   }
-  m.opts.tileLog2 = kTileLog;
+  if (m.opts.tileLog2 == 0) m.opts.tileLog2 = kTileLog;
+  if (m.opts.tileLog2 < 8 || m.opts.tileLog2 > 11) {
+    setError("LBVH_allocateWorkingMemory: tileLog2 must be in [8, 11] (got %u)", m.opts.tileLog2);
+    return m;
+  }
   m.opts.digitBits = kSortRadixBits;
   if (!validateMesh(mesh, "LBVH_allocateWorkingMemory")) return m;
   const uint32_t n = mesh->numFaces;

@@ -190,3 +190,63 @@ void bvht_sceneSpheresSoup(uint32_t numSpheres, uint32_t numSoup, uint32_t seed,
 }
 
 }  // extern "C"
+
+extern "C" {
+
+void bvht_sceneBatch(uint32_t numMeshes, uint32_t seed, uint32_t minTris, uint32_t maxTris, uint32_t *faceCounts,
+                     uint32_t *vertCounts, bvht_vec3 *vert, bvht_Triangle *faces, uint64_t *totalVert,
+                     uint64_t *totalFaces) {
+  // mesh i: seed + i; triangle target exp(U(ln minTris, ln maxTris)); kind i % 3
+  static const Icosphere ico2 = makeIcosphere(2);  // 320 faces, 162 vertices
+  const double l0 = std::log((double)minTris), l1 = std::log((double)maxTris);
+  uint64_t tv = 0, tf = 0;
+  for (uint32_t i = 0; i < numMeshes; ++i) {
+    const uint32_t s = seed + i;
+    const double target = std::exp(l0 + (l1 - l0) * hash01(s, 40, 0));
+    uint32_t nv = 0, nf = 0, gx = 1, gz = 1, spheres = 1;
+    switch (i % 3) {
+      case 0:  // heightfield with about `target` triangles (2 per cell)
+        gz = (uint32_t)std::max(1.0, std::round(std::sqrt(target / 4.0)));
+        gx = (uint32_t)std::max(1.0, std::round(target / (2.0 * gz)));
+        nv = (gx + 1) * (gz + 1);
+        nf = 2 * gx * gz;
+        break;
+      case 1:  // cluster of subdivision-2 icospheres
+        spheres = (uint32_t)std::max(1.0, std::round(target / 320.0));
+        nv = spheres * (uint32_t)ico2.v.size();
+        nf = spheres * (uint32_t)ico2.f.size();
+        break;
+      default:  // soup
+        nf = (uint32_t)std::max(1.0, std::round(target));
+        nv = 3 * nf;
+        break;
+    }
+    if (faceCounts) faceCounts[i] = nf;
+    if (vertCounts) vertCounts[i] = nv;
+    if (vert && faces) {
+      bvht_vec3 *v = vert + tv;
+      bvht_Triangle *f = faces + tf;
+      if (i % 3 == 0) {
+        bvht_sceneHeightfield(gx, gz, s, 0.2f, v, f, nullptr, nullptr);
+      } else if (i % 3 == 1) {
+        for (uint32_t k = 0; k < spheres; ++k) {
+          const double r = 0.02 + 0.08 * hash01(s, 41, k);
+          const double cx = hash01(s, 42, 3 * k), cy = hash01(s, 42, 3 * k + 1), cz = hash01(s, 42, 3 * k + 2);
+          const uint32_t vo = k * (uint32_t)ico2.v.size(), fo = k * (uint32_t)ico2.f.size();
+          for (size_t q = 0; q < ico2.v.size(); ++q)
+            v[vo + q] = tof(cx + r * ico2.v[q].x, cy + r * ico2.v[q].y, cz + r * ico2.v[q].z);
+          for (size_t q = 0; q < ico2.f.size(); ++q)
+            f[fo + q] = {ico2.f[q].v1 + vo, ico2.f[q].v2 + vo, ico2.f[q].v3 + vo};
+        }
+      } else {
+        bvht_sceneSoup(nf, s, 0.01f, v, f, nullptr, nullptr);
+      }
+    }
+    tv += nv;
+    tf += nf;
+  }
+  if (totalVert) *totalVert = tv;
+  if (totalFaces) *totalFaces = tf;
+}
+
+}  // extern "C"

@@ -82,8 +82,11 @@ __device__ __forceinline__ bvht_vec3 vcross(bvht_vec3 x, bvht_vec3 y) {
   return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
 }
 __device__ __forceinline__ float vdot(bvht_vec3 a, bvht_vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// glm::normalize = v * (1 / sqrt(dot(v, v))), both operations correctly
+// rounded.  sqrtf (not __fsqrt_rn, which maps to the 1-ulp v_sqrt_f32 on this
+// toolchain) lowers to v_sqrt_f32 plus an fma residual correction.
 __device__ __forceinline__ bvht_vec3 vnormalize(bvht_vec3 v) {
-  float s = __fdiv_rn(1.0f, __fsqrt_rn(vdot(v, v)));
+  float s = __fdiv_rn(1.0f, sqrtf(vdot(v, v)));
   return {v.x * s, v.y * s, v.z * s};
 }
 
@@ -263,8 +266,8 @@ __global__ __launch_bounds__(256) void k_bounce(const bvht_Ray *__restrict__ pri
   const uint64_t h = mix64(((uint64_t)seed << 32) ^ (uint64_t)i);
   const float u1 = (float)(h >> 40) * (1.0f / 16777216.0f);
   const float u2 = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);
-  const float rr = __fsqrt_rn(u1), phi = 6.28318530717958647692f * u2;
-  const float lx = rr * cosf(phi), ly = rr * sinf(phi), lz = __fsqrt_rn(fmaxf(0.0f, 1.0f - u1));
+  const float rr = sqrtf(u1), phi = 6.28318530717958647692f * u2;
+  const float lx = rr * cosf(phi), ly = rr * sinf(phi), lz = sqrtf(fmaxf(0.0f, 1.0f - u1));
   const float sg = n.z >= 0.0f ? 1.0f : -1.0f;
   const float a = -1.0f / (sg + n.z), b = n.x * n.y * a;
   const bvht_vec3 t1 = {1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x};

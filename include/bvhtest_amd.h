@@ -245,6 +245,41 @@ void bvht_LBVH_setTiming(bool enable);
 int bvht_LBVH_stageTimes(float *ms, int maxStages);
 const char *bvht_LBVH_stageName(int i);
 
+/* Autotuner (extension): times tile sizes 2^8..2^11 of the build on the
+ * caller's input and keeps the fastest in mem->opts.tileLog2 (every choice
+ * gives the same tree).  msOut (4 floats, optional) receives the mean times.
+ * Returns the chosen log2. */
+int bvht_LBVH_autotune(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh,
+                       float *msOut);
+
+/* Batched (segmented) build of many meshes (extension; the reference builds
+ * one mesh per LBVH command).  Meshes are concatenated: mesh i owns faces
+ * [faceOffset_i, +faceCounts[i]) whose indices are local to its vertices
+ * [vertOffset_i, +vertCounts[i]), and writes its 2n-1 nodes at nodeOffset_i
+ * with local node and face indices -- exactly the standalone build of that
+ * mesh (30-bit keys, reference topology). */
+typedef struct bvht_LBVH_BatchWorkingMemory {
+  bool lRes;
+  uint32_t numMeshes;
+  uint32_t totalFaces;
+  uint32_t totalVert;
+  uint32_t totalNodes;
+  uint32_t tileLog2;
+  void *internal;
+} bvht_LBVH_BatchWorkingMemory;
+
+bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const uint32_t *faceCounts,
+                                                     const uint32_t *vertCounts, uint32_t tileLog2);
+/* useGraph: capture the launch sequence into a hipGraph on the first call and
+ * replay it afterwards (re-captured when the pointers change). */
+bool bvht_LBVH_buildBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *vert, const bvht_Triangle *faces,
+                          bvht_BVHNode *nodes, bool useGraph);
+void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem);
+bool bvht_LBVH_batchOffsets(const bvht_LBVH_BatchWorkingMemory *mem, uint32_t i, uint32_t *faceOffset,
+                            uint32_t *vertOffset, uint32_t *nodeOffset);
+int bvht_LBVH_autotuneBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *vert, const bvht_Triangle *faces,
+                            bvht_BVHNode *nodes, float *msOut);
+
 /* ------------------------------------------------------------------------ */
 /* Bittner13 GPU optimizer -- lib/builder/Bittner13CUDA.hpp:56-83           */
 /* ------------------------------------------------------------------------ */
@@ -304,7 +339,9 @@ uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data);
  * [0] lock conflicts, [1] patches dropped (removal impossible or reinsertion
  * at the root), [2] patches dropped because their insertion point lies in a
  * subtree another patch moves, [3] search frontier overflows, [4] applied
- * patches.  Returns the number of words written. */
+ * patches, [5] patches the conflict resolution left undecided (0 unless the
+ * device breaks memory ordering; they are dropped).  Returns the number of
+ * words written (at most 6). */
 int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords);
 /* Weighted SAH of the device tree (lib/base/BVH.cpp:23-37 semantics:
  * sum over non-root nodes of SA * (leaf ? cLeaf : cInner) / SA(root)),
@@ -357,6 +394,13 @@ void bvht_sceneHeightfield(uint32_t gx, uint32_t gz, uint32_t seed, float amplit
  * (buffers NULL), then fill. */
 void bvht_sceneSpheresSoup(uint32_t numSpheres, uint32_t numSoup, uint32_t seed, bvht_vec3 *vert,
                            bvht_Triangle *faces, uint32_t *numVert, uint32_t *numFaces);
+/* Batch of meshes: mesh i (seed + i) has about exp(U(ln minTris, ln maxTris))
+ * triangles; kind i mod 3 = heightfield, icosphere cluster, triangle soup.
+ * Pass NULL buffers to get the counts (faceCounts/vertCounts per mesh and the
+ * totals), then call again with buffers of those sizes. */
+void bvht_sceneBatch(uint32_t numMeshes, uint32_t seed, uint32_t minTris, uint32_t maxTris, uint32_t *faceCounts,
+                     uint32_t *vertCounts, bvht_vec3 *vert, bvht_Triangle *faces, uint64_t *totalVert,
+                     uint64_t *totalFaces);
 /* Uniform triangle soup in the unit cube. */
 void bvht_sceneSoup(uint32_t numTris, uint32_t seed, float size, bvht_vec3 *vert, bvht_Triangle *faces,
                     uint32_t *numVert, uint32_t *numFaces);
