@@ -406,10 +406,10 @@ class Bittner13GPU:
         return sahs
 
     def stats(self):
-        out = (ctypes.c_uint32 * 6)()
-        lib().bvht_bn13_stats(ctypes.byref(self.mem), out, 6)
+        out = (ctypes.c_uint32 * 7)()
+        lib().bvht_bn13_stats(ctypes.byref(self.mem), out, 7)
         return dict(conflicts=out[0], failed=out[1], contained=out[2], overflows=out[3], applied=out[4],
-                    unresolved=out[5], skipped=lib().calcNumSkipped(ctypes.byref(self.mem)))
+                    unresolved=out[5], runaway=out[6], skipped=lib().calcNumSkipped(ctypes.byref(self.mem)))
 
     def free(self):
         if self.mem is not None and self.mem.result:

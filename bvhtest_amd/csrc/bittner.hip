@@ -43,7 +43,7 @@ constexpr int kStackCap = 1024;  // per-search frontier (findNode1 heap: 4096)
 constexpr int kStackCapAlt = 16; // altFindNode: bounded frontier (findNode2 keeps 16 slots)
 
 // Stats words (device)
-enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_UNRESOLVED, ST_WORDS = 8 };
+enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_UNRESOLVED, ST_RUNAWAY, ST_WORDS = 8 };
 
 // Patch states and node locks.  A lock holds key(round, rank): the upper word
 // is ~round, so later rounds and applied patches (round kRoundWin, upper word
@@ -272,10 +272,15 @@ __device__ void spliceAt(Patch &P, const bvht_BVHNode *nodes, uint32_t node, uin
 //
 // Exact: an entry is expanded only while induced + SA(x) < best, which is a
 // lower bound of the cost of every node below it, so the returned node is a
-// cost minimiser of the patched view (ties: smallest total, then smallest
-// node id).  Frontier: LDS stack, up to 64 entries evaluated per step.
+// cost minimiser of the patched view (ties: the first step that reaches the
+// minimum, smallest node id inside it).  Frontier: LDS stack, up to 64
+// entries evaluated per step.  Every step pops at least one entry and a tree
+// pushes each node at most once, so a search of a tree ends within `maxSteps`
+// = node count steps; a search still running then (a patched view that is
+// not a tree) returns NONE and its patch is dropped.
 __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ nodes, uint32_t x, uint32_t root,
-                               uint32_t *s_node, float *s_cost, uint32_t *s_top, int cap, uint32_t *overflow) {
+                               uint32_t *s_node, float *s_cost, uint32_t *s_top, int cap, uint32_t maxSteps,
+                               uint32_t *overflow) {
   const uint32_t lane = threadIdx.x;
   float xb[6];
   globalBox(nodes, x, xb);  // getOrig(_n)->bbox
@@ -290,7 +295,8 @@ __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ 
   uint32_t bestNode = 0;
   const uint64_t lt = __lanemask_lt();
   uint32_t top = 1;
-  while (top > 0) {
+  for (uint32_t step = 0; top > 0; ++step) {
+    if (step >= maxSteps) return NONE;  // uniform: step and top are wave-uniform
     const uint32_t m = top < 64 ? top : 64;
     const bool active = lane < m;
     uint32_t node = 0;
@@ -350,7 +356,7 @@ __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ 
   return bestNode;
 }
 
-__global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ nodes, uint32_t root,
+__global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ nodes, uint32_t root, uint32_t N,
                                                const uint32_t *__restrict__ sortedIds, uint32_t chunk,
                                                uint32_t numChunks, uint32_t K, uint32_t offsetAccess, int cap,
                                                PatchRec *__restrict__ recs, uint64_t *locks,
@@ -369,9 +375,10 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
   }
   __syncthreads();
   if (s_ok) {
-    const uint32_t b1 = waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, cap, stats + ST_OVERFLOW);
+    const uint32_t b1 = waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW);
     if (lane == 0) {
-      if (b1 == root) {
+      if (b1 == root || b1 == NONE) {
+        if (b1 == NONE) atomicAdd(stats + ST_RUNAWAY, 1u);
         s_ok = 0;
       } else {
         s_best[0] = b1;
@@ -381,9 +388,10 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
     __syncthreads();
   }
   if (s_ok) {
-    const uint32_t b2 = waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, cap, stats + ST_OVERFLOW);
+    const uint32_t b2 = waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW);
     if (lane == 0) {
-      if (b2 == root) {
+      if (b2 == root || b2 == NONE) {
+        if (b2 == NONE) atomicAdd(stats + ST_RUNAWAY, 1u);
         s_ok = 0;
       } else {
         s_best[1] = b2;
@@ -604,12 +612,17 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
   uint32_t node = starts[i];
   if (node == NONE) return;
   for (uint32_t steps = 0; steps < N; ++steps) {
-    if (__hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) != 1) return;
+    // release of the boxes written on the previous trip: fence after the
+    // stores, then a relaxed count (see fenceReleaseAgent)
+    const uint32_t left = __hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (left != 1) return;
+    fenceAcquireAgent();
     const uint32_t l = nodes[node].left, r = nodes[node].right;
     bvht_AABB b = nodes[l].bbox;
     mergeInto(b, nodes[r].bbox);
     nodes[node].bbox = b;
     nodes[node].surfaceArea = surfaceArea(b);
+    fenceReleaseAgent();
     const uint32_t p = nodes[node].parent;
     if (p == node) return;
     node = p;
@@ -956,7 +969,7 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
     return;
   }
   const int cap = cfg.altFindNode ? kStackCapAlt : kStackCap;
-  k_search<<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->sorted, chunk, cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u,
+  k_search<<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, in->sorted, chunk, cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u,
                                    cap, in->recs, in->locks, in->stats);
   if (!checkLaunch("k_search")) return;
   k_commitFirst<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats,
@@ -1020,7 +1033,8 @@ uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data) {
 
 // Counters of the optimizer (extension): conflicts, failed patches,
 // containment drops, frontier overflows, applied patches, patches left
-// undecided by the conflict resolution (always 0).
+// undecided by the conflict resolution (always 0), searches stopped at the
+// step bound (always 0 for a tree).
 int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
   Internal *in = inOf(data);
   if (!in || !out) return 0;
@@ -1028,7 +1042,7 @@ int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
   if (!check(hipMemcpyAsync(h, in->stats, sizeof(h), hipMemcpyDeviceToHost, stream()), "bn13 stats") ||
       !check(hipStreamSynchronize(stream()), "bn13 stats sync"))
     return 0;
-  int n = maxWords < 6 ? maxWords : 6;
+  int n = maxWords < 7 ? maxWords : 7;
   for (int i = 0; i < n; ++i) out[i] = h[i];
   return n;
 }

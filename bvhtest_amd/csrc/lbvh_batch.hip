@@ -148,6 +148,45 @@ bool batchStages(BatchInternal *in, const bvht_vec3 *vert, const bvht_Triangle *
   return false;
 }
 
+void freeTiles(BatchInternal *in) {
+  for (auto *t : in->tiles) devFree(t);
+  in->tiles.clear();
+  in->tileCounts.clear();
+}
+
+// Tile tables per level for tile log2 `tlog`: level 0 lists every mesh,
+// level L >= 1 only the meshes whose previous level left more than one tile
+// root.  Every (mesh, group) pair satisfies group << tlog < units of the level.
+bool buildTiles(BatchInternal *in, int tlog) {
+  freeTiles(in);
+  for (int L = 0;; ++L) {
+    std::vector<uint2> t;
+    const uint32_t unitLog = (uint32_t)(L * tlog);
+    bool more = false;
+    for (uint32_t i = 0; i < in->M; ++i) {
+      const uint64_t n = in->segsHost[i].n;
+      const uint64_t U = unitLog >= 40 ? 1 : (n + (1ull << unitLog) - 1) >> unitLog;
+      if (L > 0 && U <= 1) continue;
+      const uint64_t G = (U + (1ull << tlog) - 1) >> tlog;
+      for (uint64_t g = 0; g < G; ++g) t.push_back(make_uint2(i, (uint32_t)g));
+      if (G > 1) more = true;
+    }
+    uint2 *d = nullptr;
+    if (!t.empty()) {
+      if (!devAlloc(&d, t.size(), "batch tiles") ||
+          !check(hipMemcpy(d, t.data(), sizeof(uint2) * t.size(), hipMemcpyHostToDevice), "batch tiles")) {
+        devFree(d);
+        return false;
+      }
+    }
+    in->tiles.push_back(d);
+    in->tileCounts.push_back((uint32_t)t.size());
+    if (!more) break;
+  }
+  in->tlog = tlog;
+  return true;
+}
+
 void dropGraph(BatchInternal *in) {
   if (in->exec) (void)hipGraphExecDestroy(in->exec);
   if (in->graph) (void)hipGraphDestroy(in->graph);
@@ -208,32 +247,9 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
   in->totalFaces = (uint32_t)fo;
   in->totalVert = (uint32_t)vo;
   in->totalNodes = (uint32_t)no;
-  // tile tables per level: level 0 lists every mesh, level L >= 1 only the
-  // meshes whose previous level left more than one tile root
-  for (int L = 0;; ++L) {
-    std::vector<uint2> t;
-    const uint32_t unitLog = (uint32_t)(L * in->tlog);
-    bool more = false;
-    for (uint32_t i = 0; i < numMeshes; ++i) {
-      const uint64_t n = faceCounts[i];
-      const uint64_t U = unitLog >= 40 ? 1 : (n + (1ull << unitLog) - 1) >> unitLog;
-      if (L > 0 && U <= 1) continue;
-      const uint64_t G = (U + (1ull << in->tlog) - 1) >> in->tlog;
-      for (uint64_t g = 0; g < G; ++g) t.push_back(make_uint2(i, (uint32_t)g));
-      if (G > 1) more = true;
-    }
-    uint2 *d = nullptr;
-    if (!t.empty()) {
-      if (!devAlloc(&d, t.size(), "batch tiles") ||
-          !check(hipMemcpy(d, t.data(), sizeof(uint2) * t.size(), hipMemcpyHostToDevice), "batch tiles")) {
-        devFree(d);
-        bvht_LBVH_freeBatch(&m);
-        return m;
-      }
-    }
-    in->tiles.push_back(d);
-    in->tileCounts.push_back((uint32_t)t.size());
-    if (!more) break;
+  if (!buildTiles(in, in->tlog)) {
+    bvht_LBVH_freeBatch(&m);
+    return m;
   }
   int segBits = 1;
   while ((1u << segBits) < numMeshes) segBits++;
@@ -274,8 +290,8 @@ bool bvht_LBVH_buildBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *ve
       setError("bvht_LBVH_buildBatch: tileLog2 must be in [8, 11]");
       return false;
     }
-    in->tlog = (int)mem->tileLog2;
     dropGraph(in);
+    if (!buildTiles(in, (int)mem->tileLog2)) return false;  // tables are per tile size
   }
   if (!useGraph) return batchStages(in, vert, faces, nodes);
   if (in->exec && (in->capVert != vert || in->capFaces != faces || in->capNodes != nodes)) dropGraph(in);
@@ -309,7 +325,7 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem) {
   if (in) {
     dropGraph(in);
     if (in->capStream) (void)hipStreamDestroy(in->capStream);
-    for (auto *t : in->tiles) devFree(t);
+    freeTiles(in);
     devFree(in->segs);
     devFree(in->boxOrd);
     devFree(in->keys[0]);

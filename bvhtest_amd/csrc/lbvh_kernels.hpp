@@ -80,6 +80,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__rest
   const bvht_vec3 *__restrict__ vert = vertAll + sd.vertBase;
   const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
   const uint64_t u0 = (uint64_t)group << TLOG;
+  if (u0 >= U) return;  // block-uniform: a group past the last unit owns nothing
   const uint32_t cnt = (uint32_t)min((uint64_t)T, U - u0);
 
   for (uint32_t k = threadIdx.x; k < cnt; k += kBuildBlock) {

@@ -17,6 +17,15 @@ __device__ __forceinline__ uint32_t loadRelaxedAgent(const uint32_t *p) {
   return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Agent-scope fences for publishing data through a relaxed counter RMW.
+// In a loop that stores and then increments a counter on the next trip, an
+// acq_rel RMW at the loop head is lowered here to `buffer_wbl2` + the atomic
+// without waiting for the stores of the previous trip (vmcnt), so a reader
+// that sees the count can still read the old data.  A release fence placed
+// right after the stores lowers to `buffer_wbl2 sc1; s_waitcnt vmcnt(0)`.
+__device__ __forceinline__ void fenceReleaseAgent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fenceAcquireAgent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
 // Inclusive scan across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t x) {
   const uint32_t lane = threadIdx.x & 63;

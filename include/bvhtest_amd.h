@@ -340,8 +340,9 @@ uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data);
  * at the root), [2] patches dropped because their insertion point lies in a
  * subtree another patch moves, [3] search frontier overflows, [4] applied
  * patches, [5] patches the conflict resolution left undecided (0 unless the
- * device breaks memory ordering; they are dropped).  Returns the number of
- * words written (at most 6). */
+ * device breaks memory ordering; they are dropped), [6] searches stopped at
+ * the step bound (0 for a tree; the patch is dropped).  Returns the number of
+ * words written (at most 7). */
 int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords);
 /* Weighted SAH of the device tree (lib/base/BVH.cpp:23-37 semantics:
  * sum over non-root nodes of SA * (leaf ? cLeaf : cInner) / SA(root)),

@@ -20,6 +20,8 @@
 //  (3) the heap of findNode has capacity checks (the reference asserts).
 #include "oracle.hpp"
 
+#include <cstdio>
+#include <cstdlib>
 #include <tuple>
 #include <utility>
 
@@ -311,6 +313,89 @@ struct GpuRef {
     return bestIdx;
   }
 
+  // The device search of this project's Bittner13 kernel, restated step for
+  // step (not a reference algorithm): an exact branch and bound over a LIFO
+  // frontier, taking up to 64 entries per step (entry top-1-j goes to lane
+  // j); the step minimum replaces the best cost when strictly smaller, ties
+  // inside the step go to the smallest node id; children are pushed in lane
+  // order, the frontier is truncated at `waveCap` entries.  Used to replay the
+  // device trajectory on the CPU (searchMode 1 and 2).
+  int searchMode = 0;
+  uint32_t waveCap = 1024;
+  uint64_t waveOverflows = 0, waveRunaway = 0;
+  uint64_t waveIters = 0, waveMaxIters = 0, chunkMaxIters = 0, sumChunkMax = 0;
+  std::vector<uint32_t> sNode = std::vector<uint32_t>(1024);
+  std::vector<float> sCost = std::vector<float>(1024);
+  uint32_t findNodeWave(uint32_t n, const Patch &p) {
+    const Node &x = p.getOrig(n);
+    const Box xbox = x.bbox;
+    const float sa = x.surfaceArea;
+    sNode[0] = root;
+    sCost[0] = 0.0f;
+    uint32_t top = 1;
+    float best = HUGE_VALF;
+    uint32_t bestNode = 0;
+    uint32_t lnode[64], left[64], right[64];
+    float total[64], newInd[64];
+    bool eval[64], leaf[64];
+    // the device bound: one step per node (each step pops one entry or more,
+    // a tree pushes each node once); a view that is not a tree fails the search
+    const uint64_t limit = b.size();
+    for (uint64_t it = 0; top > 0; ++it) {
+      if (it >= limit) {
+        waveRunaway++;
+        return NONE;
+      }
+      waveIters++;
+      if (it + 1 > chunkMaxIters) chunkMaxIters = it + 1;
+      if (it + 1 > waveMaxIters) waveMaxIters = it + 1;
+      const uint32_t m = top < 64 ? top : 64;
+      const uint32_t newTop = top - m;
+      float mt = HUGE_VALF;
+      for (uint32_t j = 0; j < m; ++j) {
+        lnode[j] = sNode[top - 1 - j];
+        const float ind = sCost[top - 1 - j];
+        eval[j] = (ind + sa) < best;
+        total[j] = HUGE_VALF;
+        if (!eval[j]) continue;
+        PNode cn = p.getSubset(lnode[j]);
+        Box box = p.getAABB(lnode[j]);
+        float ni = -1.0f * boxSA(box);
+        mergeWith(box, xbox);
+        total[j] = ind + boxSA(box);
+        newInd[j] = ni + total[j];
+        leaf[j] = cn.isLeafFlag != 0;
+        left[j] = cn.left;
+        right[j] = cn.right;
+        if (total[j] < mt) mt = total[j];
+      }
+      if (mt < best) {
+        uint32_t cand = NONE;
+        for (uint32_t j = 0; j < m; ++j)
+          if (eval[j] && total[j] == mt && lnode[j] < cand) cand = lnode[j];
+        best = mt;
+        bestNode = cand;
+      }
+      uint32_t pos = newTop;
+      for (uint32_t j = 0; j < m; ++j) {
+        if (!(eval[j] && !leaf[j] && (newInd[j] + sa) < best)) continue;
+        if (pos + 1 < waveCap) {
+          sNode[pos] = left[j];
+          sCost[pos] = newInd[j];
+          sNode[pos + 1] = right[j];
+          sCost[pos + 1] = newInd[j];
+        }
+        pos += 2;
+      }
+      if (pos > waveCap) {
+        waveOverflows++;
+        pos = waveCap & ~1u;
+      }
+      top = pos;
+    }
+    return bestNode;
+  }
+
   // removeNode -- Bittner13CUDA.cu:267-305.  Returns false without patching
   // anything when the node cannot be removed (deviation (1)).
   bool removeNode(uint32_t node, Patch &p, uint32_t ins[2], uint32_t unused[2], uint32_t gpSib[2]) {
@@ -349,8 +434,8 @@ struct GpuRef {
 
   // reinsert -- Bittner13CUDA.cu:308-358 (findNode1 path).
   bool reinsert(uint32_t node, uint32_t unusedIdx, Patch &p, bool update, uint32_t &bestOut) {
-    uint32_t bestIdx = findNode1(node, p);
-    if (bestIdx == root) return false;
+    uint32_t bestIdx = searchMode ? findNodeWave(node, p) : findNode1(node, p);
+    if (bestIdx == root || bestIdx == NONE) return false;
     uint32_t bp = p.patchIndex(bestIdx);
     PNode *best;
     if (bp == NONE)
@@ -490,14 +575,19 @@ int orc_bittner13_seq(Node *nodes, uint32_t N, uint32_t *root, float batchPercen
 //   first step and after each step (f64 sum).
 //   stats[0] = skipped (lock conflicts), [1] = patches dropped by the
 //   reinsert-at-root rule or a failed removal, [2] = patches dropped by
-//   deviation (2), [3] = heap overflows.
+//   deviation (2), [3] = heap / frontier overflows, [4] = replayed device
+//   searches stopped because the patched view was not a tree.
+//   chunkTrees (optional, steps * numChunks * N nodes): the tree after every
+//   chunk.
 // Returns 0, or 2 if the tree became invalid.
 int orc_bittner13_gpuref(Node *nodes, uint32_t N, float batchPercent, uint32_t numChunks, uint32_t steps,
-                         int offsetAccess, double *sahOut, uint64_t *stats) {
+                         int offsetAccess, int searchMode, Node *chunkTrees, double *sahOut, uint64_t *stats) {
   if (batchPercent <= 0.01f) batchPercent = 0.01f;
   if (batchPercent >= 75.0f) batchPercent = 75.0f;
   std::vector<Node> b(nodes, nodes + N);
   GpuRef g{b};
+  g.searchMode = searchMode;
+  g.waveCap = searchMode == 2 ? 16 : 1024;
   // Bittner13GPU::runImpl -- Bittner13GPU.cpp:682-708
   uint32_t numNodes = (uint32_t)((batchPercent / 100.0f) * (float)N);
   uint32_t K = numNodes / numChunks;
@@ -512,7 +602,7 @@ int orc_bittner13_gpuref(Node *nodes, uint32_t N, float batchPercent, uint32_t n
   std::vector<uint32_t> bestOf(2 * (size_t)K);
   std::vector<uint32_t> movedOf(2 * (size_t)K);
   std::vector<uint8_t> ok(K);
-  uint64_t st[4] = {0, 0, 0, 0};
+  uint64_t st[5] = {0, 0, 0, 0, 0};
   auto sahGpu = [&]() {
     double s = 0.0;
     for (uint32_t i = 0; i < N; ++i) s += b[i].surfaceArea;
@@ -627,11 +717,19 @@ int orc_bittner13_gpuref(Node *nodes, uint32_t N, float batchPercent, uint32_t n
         return 2;
       }
       g.fixTree1();
+      g.sumChunkMax += g.chunkMaxIters;
+      g.chunkMaxIters = 0;
+      if (chunkTrees) std::copy(b.begin(), b.end(), chunkTrees + ((size_t)s * numChunks + c) * N);
     }
     if (sahOut) sahOut[s + 1] = sahGpu();
   }
-  st[3] = g.heapOverflows;
-  if (stats) std::copy(st, st + 4, stats);
+  st[3] = g.searchMode ? g.waveOverflows : g.heapOverflows;
+  st[4] = g.waveRunaway;
+  if (std::getenv("ORC_WAVE_TRACE"))
+    std::fprintf(stderr, "wave iterations: total %llu, max per search %llu, sum of per-chunk maxima %llu\n",
+                 (unsigned long long)g.waveIters, (unsigned long long)g.waveMaxIters,
+                 (unsigned long long)g.sumChunkMax);
+  if (stats) std::copy(st, st + 5, stats);
   std::copy(b.begin(), b.end(), nodes);
   return 0;
 }

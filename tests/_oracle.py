@@ -35,7 +35,7 @@ _SIG = {
     "orc_trace": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f, _vp, _vp, _vp]),
     "orc_trace_bruteforce": (None, [_vp, _u32, _vp, _vp, _u32, ctypes.c_int, _f, _vp]),
     "orc_bittner13_seq": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(_u32), _f, _u32, ctypes.c_int, _vp, _vp]),
-    "orc_bittner13_gpuref": (ctypes.c_int, [_vp, _u32, _f, _u32, _u32, ctypes.c_int, _vp, _vp]),
+    "orc_bittner13_gpuref": (ctypes.c_int, [_vp, _u32, _f, _u32, _u32, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
     "orc_tree_is_valid": (ctypes.c_int, [_vp, _u32, _u32]),
     "orc_seq_find_node_box": (_u32, [_vp, _u32, _u32, _vp, _f]),
     "orc_lbvh_omp": (ctypes.c_double, [_vp, _vp, _u32, ctypes.c_int, _vp, _vp, ctypes.c_int]),
@@ -132,12 +132,20 @@ def bittner_seq(nodes, root, batch_percent, steps, sort=True):
     return nodes, r.value, sah
 
 
-def bittner_gpuref(nodes, batch_percent, num_chunks, steps, offset_access=False):
+def bittner_gpuref(nodes, batch_percent, num_chunks, steps, offset_access=False, search=0, chunk_trees=False):
+    """search: 0 = the reference's heap search (findNode1), 1 = replay of the
+    device wave search (frontier 1024), 2 = the same with frontier 16.
+    stats: skipped (lock conflicts), dropped, contained, overflows, runaway
+    searches.  With chunk_trees the tree after every chunk is returned too
+    (array of shape (steps * num_chunks, N))."""
     nodes = nodes.copy()
     sah = np.empty(steps + 1, np.float64)
-    stats = np.zeros(4, np.uint64)
+    stats = np.zeros(5, np.uint64)
+    trees = np.zeros((steps * num_chunks, len(nodes)), NODE_DTYPE) if chunk_trees else None
     rc = lib().orc_bittner13_gpuref(p(nodes), len(nodes), batch_percent, num_chunks, steps, int(offset_access),
-                                    p(sah), p(stats))
+                                    int(search), p(trees) if chunk_trees else None, p(sah), p(stats))
+    if chunk_trees:
+        return rc, nodes, sah, stats, trees
     return rc, nodes, sah, stats
 
 
