@@ -163,6 +163,9 @@ def run_bench(args):
         btree.free()
         print(f"# bittner: {bittner}", file=sys.stderr, flush=True)
 
+    batch = None if args.no_batch else batch_bench(bt, L)
+    learned = None if args.no_learned else learned_bench(bt)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(v, f, bits)
@@ -174,8 +177,70 @@ def run_bench(args):
                                   "1 diffuse bounce", "triangles": n, "spheres": 2048, "soup": 500000,
                       "morton_bits": bits, "topology": "reference (index Karras)", "seed": 2},
            "roofline": roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
-           "trace": trace, "bittner": bittner, "cpu_baseline": cpu}
+           "trace": trace, "bittner": bittner, "batched": batch, "learned_priority": learned, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
+
+
+def batch_bench(bt, L, reps=5):
+    """512 meshes of log-uniform size in [1K, 500K] triangles: one segmented
+    build replayed from a hipGraph (tile size autotuned) vs. 512 sequential
+    single-mesh builds of the same meshes (30-bit keys both)."""
+    t0 = time.time()
+    v, f, fc, vc = bt.scene_batch(512, 1000, 1000, 500000)
+    total = int(fc.sum())
+    print(f"# batch scene: {len(fc)} meshes, {total} triangles ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    b = bt.BatchBuilder(v, f, fc, vc)
+    best, ms = b.autotune()
+    b.build(use_graph=True)
+    L.LBVH_doCUDASyc()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.build(use_graph=True)
+    L.LBVH_doCUDASyc()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the same meshes one by one, mesh data and trees already on the device
+    meshes, builders, trees = [], [], []
+    for i in range(len(fc)):
+        fo, vo, _ = b.offsets(i)
+        mv = v[vo:vo + int(vc[i])]
+        mf = f[fo:fo + int(fc[i])]
+        m = bt.DeviceMesh(mv, mf)
+        lb = bt.LBVHBuilder(m, 30)
+        meshes.append(m)
+        builders.append(lb)
+        trees.append(lb.allocate_bvh())
+    for lb, tr in zip(builders, trees):
+        lb.build(tr)
+    L.LBVH_doCUDASyc()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for lb, tr in zip(builders, trees):
+            lb.build(tr)
+    L.LBVH_doCUDASyc()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / reps
+    for lb in builders:
+        lb.free()
+    out = {"meshes": int(len(fc)), "triangles": total, "tile_log2": int(best),
+           "autotune_ms": [round(x, 3) for x in ms], "graph_ms": round(graph_ms, 3),
+           "graph_mtri_per_s": round(total / graph_ms / 1e3, 1), "sequential_ms": round(seq_ms, 3),
+           "sequential_mtri_per_s": round(total / seq_ms / 1e3, 1), "speedup": round(seq_ms / graph_ms, 2)}
+    print(f"# batched: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def learned_bench(bt, seed=200, plain_steps=50):
+    """Time to reach the plain optimizer's final SAH(1.2, 1) within 0.5%:
+    learned priority vs. plain, on a held-out 4.2M-triangle scene."""
+    path = os.path.join(ROOT, "bvhtest_amd", "data", "priority_mlp.npy")
+    if not os.path.exists(path):
+        return {"status": "no trained weights at bvhtest_amd/data/priority_mlp.npy"}
+    from bvhtest_amd import priority_training as pt
+
+    model = bt.PriorityModel.load(path)
+    r = pt.evaluate(model, seed, 2048, 500000, plain_steps)
+    r["weights"] = "bvhtest_amd/data/priority_mlp.npy"
+    print(f"# learned priority: {r}", file=sys.stderr, flush=True)
+    return r
 
 
 def cpu_baseline(v, f, bits, seconds=12.0):
@@ -215,6 +280,8 @@ def main():
     ap.add_argument("--morton-bits", type=int, default=63, choices=[30, 63])
     ap.add_argument("--bittner-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "

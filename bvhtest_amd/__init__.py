@@ -348,12 +348,14 @@ class Bittner13GPU:
     """bittner13GPU command: options and defaults of lib/builder/Bittner13GPU.hpp:28-37."""
 
     def __init__(self, maxNumStepps=500, numChunks=32, CUDABlockSize=128, batchPercent=1.0, sort=True,
-                 offsetAccess=False, altFindNode=False, altFixTree=False, altSort=False, localPatchCPY=False):
+                 offsetAccess=False, altFindNode=False, altFixTree=False, altSort=False, localPatchCPY=False,
+                 priority=0):
         batchPercent = min(max(batchPercent, 0.01), 75.0)  # Bittner13GPU.cpp:652-653
         self.opts = dict(maxNumStepps=maxNumStepps, numChunks=numChunks, CUDABlockSize=CUDABlockSize,
                          batchPercent=batchPercent, sort=sort, offsetAccess=offsetAccess, altFindNode=altFindNode,
-                         altFixTree=altFixTree, altSort=altSort, localPatchCPY=localPatchCPY)
+                         altFixTree=altFixTree, altSort=altSort, localPatchCPY=localPatchCPY, priority=priority)
         self.mem = None
+        self.model = None
 
     def _sizes(self, num_nodes):
         # (uint32_t)((batchPercent / 100.0f) * (float)numNodes), chunk = that / numChunks
@@ -369,7 +371,50 @@ class Bittner13GPU:
         c.blockSize = self.opts["CUDABlockSize"]
         for k in ("offsetAccess", "altFindNode", "altFixTree", "altSort", "sort", "localPatchCPY"):
             setattr(c, k, self.opts[k])
+        c.priority = self.opts["priority"]
         return c
+
+    def set_model(self, model):
+        """Priority model for priority=1 steps (None clears)."""
+        self.model = model
+        if not lib().bvht_bn13_setModel(ctypes.byref(self.mem), model.handle if model else None):
+            _fail("bvht_bn13_setModel")
+
+    def set_priority(self, priority):
+        self.opts["priority"] = priority
+
+    def features(self, tree):
+        """(numInner, 12) features of every inner node, inner-list order."""
+        n = self.mem.numInnerNodes
+        buf = DeviceBuffer(n * 48)
+        if not lib().bvht_bn13_features(ctypes.byref(self.mem), ctypes.byref(tree.ptr), buf.ptr):
+            _fail("bvht_bn13_features")
+        return buf.download(np.float32, n * 12).reshape(n, 12)
+
+    def record(self, max_records):
+        """Start (or clear) SAH-gain recording."""
+        if not lib().bvht_bn13_setRecording(ctypes.byref(self.mem), max_records):
+            _fail("bvht_bn13_setRecording")
+
+    def records(self):
+        """(features (n, 12), gains (n,)) recorded so far."""
+        pf, pg = ctypes.c_void_p(), ctypes.c_void_p()
+        n = lib().bvht_bn13_records(ctypes.byref(self.mem), ctypes.byref(pf), ctypes.byref(pg))
+        feat = np.empty((n, 12), np.float32)
+        gain = np.empty(n, np.float32)
+        if n:
+            if not lib().runMemcpy(_ptr(feat), pf, feat.nbytes, 2) or not lib().runMemcpy(_ptr(gain), pg, gain.nbytes, 2):
+                _fail("runMemcpy")
+        return feat, gain
+
+    def probe(self, tree, positions):
+        """Record the gains of the inner nodes at `positions` (inner-list order) without changing the tree."""
+        pos = np.ascontiguousarray(positions, np.uint32)
+        buf = DeviceBuffer(max(pos.nbytes, 4)).upload(pos)
+        if not lib().bvht_bn13_probe(ctypes.byref(self.mem), ctypes.byref(tree.ptr), self.cfg(self.chunk), buf.ptr,
+                                     len(pos)):
+            _fail("bvht_bn13_probe")
+        lib().doCudaDevSync()
 
     def setup(self, tree, num_faces):
         chunk = self._sizes(tree.num_nodes)
@@ -425,6 +470,74 @@ class Bittner13GPU:
 
 def calc_sah(tree, c_inner=1.0, c_leaf=1.0):
     return lib().bvht_calcSAHWeighted(ctypes.byref(tree.ptr), c_inner, c_leaf)
+
+
+# ---------------------------------------------------------------------------
+# learned removal priority (12-64-64-1 MLP, BF16 MFMA)
+# ---------------------------------------------------------------------------
+
+class PriorityModel:
+    """Device MLP ranking Bittner13 removal candidates by predicted SAH gain."""
+
+    def __init__(self, seed=0, params=None):
+        self.handle = lib().bvht_mlpCreate(seed)
+        if not self.handle:
+            _fail("bvht_mlpCreate")
+        self.num_params = lib().bvht_mlpNumParams()
+        if params is not None:
+            self.set_params(params)
+
+    def params(self):
+        out = np.empty(self.num_params, np.float32)
+        if not lib().bvht_mlpGetParams(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+            _fail("bvht_mlpGetParams")
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (self.num_params,)
+        if not lib().bvht_mlpSetParams(self.handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+            _fail("bvht_mlpSetParams")
+
+    def predict(self, feat):
+        feat = np.ascontiguousarray(feat, np.float32).reshape(-1, 12)
+        n = len(feat)
+        fb = DeviceBuffer(max(feat.nbytes, 16)).upload(feat)
+        ob = DeviceBuffer(max(4 * n, 16))
+        if not lib().bvht_mlpPredict(self.handle, fb.ptr, n, ob.ptr):
+            _fail("bvht_mlpPredict")
+        lib().doCudaDevSync()
+        return ob.download(np.float32, n)
+
+    def train(self, feat, target, epochs=10, batch=4096, lr=1e-3, seed=0):
+        """Adam on the mean squared error; returns the last epoch's mean loss."""
+        feat = np.ascontiguousarray(feat, np.float32).reshape(-1, 12)
+        target = np.ascontiguousarray(target, np.float32)
+        assert len(feat) == len(target) and len(feat) > 0
+        fb = DeviceBuffer(feat.nbytes).upload(feat)
+        tb = DeviceBuffer(target.nbytes).upload(target)
+        loss = lib().bvht_mlpTrain(self.handle, fb.ptr, tb.ptr, len(feat), epochs, batch, lr, seed)
+        if loss < 0:
+            _fail("bvht_mlpTrain")
+        return loss
+
+    def save(self, path):
+        np.save(path, self.params())
+
+    @classmethod
+    def load(cls, path):
+        return cls(params=np.load(path, allow_pickle=False))
+
+    def free(self):
+        if self.handle:
+            lib().bvht_mlpFree(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -132,6 +132,18 @@ SIGNATURES = {
     "calcNumSkipped": (c_u32, [P(GPUWorkingMemory)]),
     "bvht_calcSAHWeighted": (ctypes.c_double, [P(CUDAMemoryBVHPointer), c_f, c_f]),
     "bvht_bn13_stats": (ctypes.c_int, [P(GPUWorkingMemory), P(c_u32), ctypes.c_int]),
+    "bvht_mlpNumParams": (c_u32, []),
+    "bvht_mlpCreate": (ctypes.c_void_p, [c_u32]),
+    "bvht_mlpFree": (None, [ctypes.c_void_p]),
+    "bvht_mlpSetParams": (ctypes.c_bool, [ctypes.c_void_p, P(c_f)]),
+    "bvht_mlpGetParams": (ctypes.c_bool, [ctypes.c_void_p, P(c_f)]),
+    "bvht_mlpPredict": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p, c_u32, ctypes.c_void_p]),
+    "bvht_mlpTrain": (c_f, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_u32, c_u32, c_u32, c_f, c_u32]),
+    "bvht_bn13_setModel": (ctypes.c_bool, [P(GPUWorkingMemory), ctypes.c_void_p]),
+    "bvht_bn13_features": (ctypes.c_bool, [P(GPUWorkingMemory), P(CUDAMemoryBVHPointer), ctypes.c_void_p]),
+    "bvht_bn13_setRecording": (ctypes.c_bool, [P(GPUWorkingMemory), c_u32]),
+    "bvht_bn13_records": (c_u32, [P(GPUWorkingMemory), P(ctypes.c_void_p), P(ctypes.c_void_p)]),
+    "bvht_bn13_probe": (ctypes.c_bool, [P(GPUWorkingMemory), P(CUDAMemoryBVHPointer), AlgoCFG, ctypes.c_void_p, c_u32]),
     "allocateRays": (c_bool, [P(c_vp), c_u32]),
     "allocateImage": (c_bool, [P(c_vp), c_u32, c_u32]),
     "freeRays": (None, [P(c_vp)]),

@@ -23,6 +23,7 @@
 //               acquire/release atomics).  Boxes are exact min/max, so this
 //               equals the reference's full refit (kFixTree1).
 // The root never moves (reinsertion at the root is refused, :310).
+#include "mlp.hpp"
 #include "radix_sort.hpp"
 #include "refit.hpp"
 #include "wave.hpp"
@@ -280,7 +281,7 @@ __device__ void spliceAt(Patch &P, const bvht_BVHNode *nodes, uint32_t node, uin
 // not a tree) returns NONE and its patch is dropped.
 __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ nodes, uint32_t x, uint32_t root,
                                uint32_t *s_node, float *s_cost, uint32_t *s_top, int cap, uint32_t maxSteps,
-                               uint32_t *overflow) {
+                               uint32_t *overflow, float &bestCost) {
   const uint32_t lane = threadIdx.x;
   float xb[6];
   globalBox(nodes, x, xb);  // getOrig(_n)->bbox
@@ -353,14 +354,40 @@ __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ 
     top = nt;
     __syncthreads();
   }
+  bestCost = best;
   return bestNode;
 }
 
+// Destination of SAH-gain records (feat == nullptr: not recording).
+struct RecordSink {
+  const float *feat;  // features by inner-list position
+  float *recFeat;     // [recMax][12]
+  float *recGain;     // [recMax]
+  uint32_t *recCount;
+  uint32_t recMax;
+};
+
+// Gain of a patch in unit-cost SAH relative to the root: the areas of the
+// two removed nodes plus the shrink of the two path boxes above the removal
+// point, minus the two insertion costs the searches found.
+__device__ float patchGain(const Patch &P, const bvht_BVHNode *nodes, uint32_t node, uint32_t parent, uint32_t root,
+                           float c1, float c2) {
+  float shrink = 0.0f;
+  for (int i = 0; i < 2; ++i)
+    if (P.path[i] != NONE) shrink += nodes[P.path[i]].surfaceArea - sa6(P.box[i]);
+  const float g = nodes[node].surfaceArea + nodes[parent].surfaceArea + shrink - c1 - c2;
+  const float sr = nodes[root].surfaceArea;
+  return sr > 0.0f ? g / sr : 0.0f;
+}
+
+// PROBE: evaluate the patches of the nodes at inner-list positions
+// order[blockIdx.x] and record their gains; nothing is reserved or applied.
+template <bool PROBE>
 __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ nodes, uint32_t root, uint32_t N,
-                                               const uint32_t *__restrict__ sortedIds, uint32_t chunk,
-                                               uint32_t numChunks, uint32_t K, uint32_t offsetAccess, int cap,
-                                               PatchRec *__restrict__ recs, uint64_t *locks,
-                                               uint32_t *__restrict__ stats) {
+                                               const uint32_t *__restrict__ order, const uint32_t *__restrict__ inner,
+                                               uint32_t numInner, uint32_t chunk, uint32_t numChunks, uint32_t K,
+                                               uint32_t offsetAccess, int cap, PatchRec *__restrict__ recs,
+                                               uint64_t *locks, uint32_t *__restrict__ stats, RecordSink sink) {
   __shared__ Patch P;
   __shared__ uint32_t s_node[kStackCap];
   __shared__ float s_cost[kStackCap];
@@ -368,14 +395,20 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
   __shared__ uint32_t s_ok, s_ins[2], s_unused[2], s_gp, s_best[2];
   const uint32_t k = blockIdx.x, lane = threadIdx.x;
   PatchRec *rec = recs + k;
-  const uint32_t nodeIdx = sortedIds[offsetAccess ? k * numChunks + chunk : chunk * K + k];
+  const uint32_t pos = PROBE ? order[k] : order[offsetAccess ? k * numChunks + chunk : chunk * K + k];
+  if (pos >= numInner) {  // block-uniform; only a bad probe list gets here
+    if (!PROBE && lane == 0) rec->ok = REC_NONE;
+    return;
+  }
+  const uint32_t nodeIdx = inner[pos];
+  float c1 = 0.0f, c2 = 0.0f;
   if (lane == 0) {
     patchClear(P);
     s_ok = removeNode(P, nodes, nodeIdx, root, s_ins, s_unused, s_gp) ? 1u : 0u;
   }
   __syncthreads();
   if (s_ok) {
-    const uint32_t b1 = waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW);
+    const uint32_t b1 = waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW, c1);
     if (lane == 0) {
       if (b1 == root || b1 == NONE) {
         if (b1 == NONE) atomicAdd(stats + ST_RUNAWAY, 1u);
@@ -388,7 +421,7 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
     __syncthreads();
   }
   if (s_ok) {
-    const uint32_t b2 = waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW);
+    const uint32_t b2 = waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW, c2);
     if (lane == 0) {
       if (b2 == root || b2 == NONE) {
         if (b2 == NONE) atomicAdd(stats + ST_RUNAWAY, 1u);
@@ -401,12 +434,20 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
     __syncthreads();
   }
   if (!s_ok) {
-    if (lane == 0) {
+    if (!PROBE && lane == 0) {
       rec->ok = REC_NONE;
       atomicAdd(stats + ST_FAILED, 1u);
     }
     return;
   }
+  if (sink.feat && lane == 0) {
+    const uint32_t slot = atomicAdd(sink.recCount, 1u);
+    if (slot < sink.recMax) {
+      for (int f = 0; f < kFeat; ++f) sink.recFeat[(size_t)slot * kFeat + f] = sink.feat[(size_t)pos * kFeat + f];
+      sink.recGain[slot] = patchGain(P, nodes, nodeIdx, s_unused[1], root, c1, c2);
+    }
+  }
+  if (PROBE) return;
   if (lane < (uint32_t)kNNode) {
     const uint32_t n = P.vPatch[lane];
     rec->node[lane] = n;
@@ -736,7 +777,31 @@ struct Internal {
   uint32_t *selCounts = nullptr;
   double *partials = nullptr;
   bool selected = false;
+  // learned priority and SAH-gain records
+  float *feat = nullptr;  // [numInner][12], inner-list order
+  uint32_t *leafCnt = nullptr;
+  float *subSA = nullptr;
+  PriorityModel *model = nullptr;
+  float *recFeat = nullptr, *recGain = nullptr;
+  uint32_t *recCount = nullptr;
+  uint32_t recMax = 0;
 };
+
+bool features(Internal *in, const bvht_CUDAMemoryBVHPointer *bvh) {
+  if (!in->feat) {
+    if (!devAlloc(&in->feat, (size_t)in->numInner * kFeat, "bn13 features") ||
+        !devAlloc(&in->leafCnt, in->N, "bn13 leaf counts") || !devAlloc(&in->subSA, in->N, "bn13 subtree SA"))
+      return false;
+  }
+  return computeFeatures(bvh->nodes, in->N, in->root, in->inner, in->numInner, in->leaves, in->numLeaves, in->leafCnt,
+                         in->subSA, in->refitCnt, in->feat);
+}
+
+RecordSink sinkOf(const Internal *in) {
+  RecordSink r{nullptr, in->recFeat, in->recGain, in->recCount, in->recMax};
+  if (in->recMax) r.feat = in->feat;
+  return r;
+}
 
 constexpr uint32_t kSahBlocks = 512;
 
@@ -857,6 +922,12 @@ void freeMemory(bvht_GPUWorkingMemory *data) {
     devFree(in->stats);
     devFree(in->selCounts);
     devFree(in->partials);
+    devFree(in->feat);
+    devFree(in->leafCnt);
+    devFree(in->subSA);
+    devFree(in->recFeat);
+    devFree(in->recGain);
+    devFree(in->recCount);
     delete in;
   }
   if (data) {
@@ -937,19 +1008,26 @@ void bn13_selectNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bv
     setError("bn13_selectNodes: topKThElement is not supported (set option sort)");
     return;
   }
+  if (cfg.priority > 1 || (cfg.priority == 1 && !in->model)) {
+    setError("bn13_selectNodes: priority %u needs %s", cfg.priority,
+             cfg.priority == 1 ? "a model (bvht_bn13_setModel)" : "0 (heuristic) or 1 (learned)");
+    return;
+  }
+  if ((cfg.priority == 1 || in->recMax) && !features(in, bvh)) return;
   uint32_t *scratch = in->sortScratch;
   if (!sortReset(in->plan, scratch)) return;
-  uint32_t blocks = divUp(in->numInner, 256 * 8);
-  if (blocks > (uint32_t)numCUs() * 8) blocks = numCUs() * 8;
-  k_cost<<<blocks, 256, 0, stream()>>>(bvh->nodes, in->inner, in->numInner, in->keys[0], sortHist(in->plan, scratch));
-  if (!checkLaunch("k_cost")) return;
+  if (cfg.priority == 1) {
+    if (!mlpInfer(*in->model, in->feat, in->numInner, in->keys[0], sortHist(in->plan, scratch), nullptr)) return;
+  } else {
+    uint32_t blocks = divUp(in->numInner, 256 * 8);
+    if (blocks > (uint32_t)numCUs() * 8) blocks = numCUs() * 8;
+    k_cost<<<blocks, 256, 0, stream()>>>(bvh->nodes, in->inner, in->numInner, in->keys[0], sortHist(in->plan, scratch));
+    if (!checkLaunch("k_cost")) return;
+  }
   void *keys[2] = {in->keys[0], in->keys[1]};
   uint32_t *vals[2] = {in->vals[0], in->vals[1]};
-  // values: node ids from the inner list (copied into vals[0])
-  if (!check(hipMemcpyAsync(in->vals[0], in->inner, 4ull * in->numInner, hipMemcpyDeviceToDevice, stream()),
-             "bn13 ids"))
-    return;
-  const int r = sortRun(in->plan, keys, vals, false, false, scratch);
+  // values: inner-list positions (identity); k_search maps them to node ids
+  const int r = sortRun(in->plan, keys, vals, true, false, scratch);
   if (r < 0) return;
   in->sorted = in->vals[r];
   in->selected = true;
@@ -969,8 +1047,9 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
     return;
   }
   const int cap = cfg.altFindNode ? kStackCapAlt : kStackCap;
-  k_search<<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, in->sorted, chunk, cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u,
-                                   cap, in->recs, in->locks, in->stats);
+  k_search<false><<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, in->sorted, in->inner, in->numInner, chunk,
+                                          cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u, cap, in->recs, in->locks,
+                                          in->stats, sinkOf(in));
   if (!checkLaunch("k_search")) return;
   k_commitFirst<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats,
                                                      in->pending[0], in->numPending);
@@ -1045,6 +1124,87 @@ int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
   int n = maxWords < 7 ? maxWords : 7;
   for (int i = 0; i < n; ++i) out[i] = h[i];
   return n;
+}
+
+// ---- learned priority and SAH-gain records (extension) -------------------
+
+bool bvht_bn13_setModel(bvht_GPUWorkingMemory *data, bvht_PriorityModel *model) {
+  Internal *in = inOf(data);
+  if (!in) {
+    setError("bvht_bn13_setModel: working memory not allocated");
+    return false;
+  }
+  in->model = reinterpret_cast<PriorityModel *>(model);
+  return true;
+}
+
+bool bvht_bn13_features(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, float *devFeat) {
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "bvht_bn13_features") || !devFeat) {
+    if (!in) setError("bvht_bn13_features: working memory not allocated");
+    if (!devFeat) setError("bvht_bn13_features: no output buffer");
+    return false;
+  }
+  if (!features(in, bvh)) return false;
+  return check(hipMemcpyAsync(devFeat, in->feat, sizeof(float) * kFeat * (size_t)in->numInner,
+                              hipMemcpyDeviceToDevice, stream()),
+               "bvht_bn13_features");
+}
+
+bool bvht_bn13_setRecording(bvht_GPUWorkingMemory *data, uint32_t maxRecords) {
+  Internal *in = inOf(data);
+  if (!in) {
+    setError("bvht_bn13_setRecording: working memory not allocated");
+    return false;
+  }
+  devFree(in->recFeat);
+  devFree(in->recGain);
+  devFree(in->recCount);
+  in->recFeat = in->recGain = nullptr;
+  in->recCount = nullptr;
+  in->recMax = 0;
+  if (maxRecords == 0) return true;
+  if (!devAlloc(&in->recFeat, (size_t)maxRecords * kFeat, "bn13 record features") ||
+      !devAlloc(&in->recGain, maxRecords, "bn13 record gains") || !devAlloc(&in->recCount, 1, "bn13 record count") ||
+      !fillU32(in->recCount, 0u, 1))
+    return false;
+  in->recMax = maxRecords;
+  return true;
+}
+
+uint32_t bvht_bn13_records(bvht_GPUWorkingMemory *data, float **devFeat, float **devGain) {
+  Internal *in = inOf(data);
+  if (!in || !in->recMax) return 0;
+  uint32_t n = 0;
+  if (!check(hipMemcpyAsync(&n, in->recCount, 4, hipMemcpyDeviceToHost, stream()), "bvht_bn13_records") ||
+      !check(hipStreamSynchronize(stream()), "bvht_bn13_records sync"))
+    return 0;
+  if (devFeat) *devFeat = in->recFeat;
+  if (devGain) *devGain = in->recGain;
+  return n < in->recMax ? n : in->recMax;
+}
+
+bool bvht_bn13_probe(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg,
+                     const uint32_t *devPositions, uint32_t num) {
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "bvht_bn13_probe")) {
+    if (!in) setError("bvht_bn13_probe: working memory not allocated");
+    return false;
+  }
+  if (!in->recMax) {
+    setError("bvht_bn13_probe: enable recording first (bvht_bn13_setRecording)");
+    return false;
+  }
+  if (num == 0) return true;
+  if (!devPositions) {
+    setError("bvht_bn13_probe: no positions");
+    return false;
+  }
+  if (!features(in, bvh)) return false;
+  const int cap = cfg.altFindNode ? kStackCapAlt : kStackCap;
+  k_search<true><<<num, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, devPositions, in->inner, in->numInner, 0, 1,
+                                          num, 0u, cap, in->recs, in->locks, in->stats, sinkOf(in));
+  return checkLaunch("k_search (probe)");
 }
 
 }  // extern "C"

@@ -1,12 +1,24 @@
-// Ray generation and stackless traversal for gfx950
+// Ray generation and traversal for gfx950
 // (reference: lib/tracer/CUDAKernels.cu:43-223, :386-417).
 //
-// Traversal is the reference's bit-stack algorithm (Afra & Szirmay-Kalos
-// 2014): descend to the nearer of the two hit children, remember the other
-// with a set bit, backtrack through parent links.  Arithmetic is f32 with no
-// FMA contraction and IEEE division, so the visit order, hit triangle and
-// distance are a pure function of the inputs (the oracle reproduces them
-// exactly).  The 128-bit stack limits trees to depth 128, as in the reference.
+// Traversal visits nodes in the reference's order (Afra & Szirmay-Kalos
+// 2014, the bit-trail algorithm of kTraceRay): at an inner node both children
+// are tested against the current nearest hit, the nearer hit child is entered
+// first and the other is remembered by a set trail bit; a remembered child is
+// entered later without a second box test.  Two changes make it fast without
+// changing that order:
+//   * each inner node's two child boxes, ids and leaf triangles are packed
+//     into one 64-byte record (k_packPairs), each triangle into (v0, e1, e2)
+//     (k_packTris, the same f32 subtractions the test makes), so a visit is
+//     one 64-byte load instead of three 48-byte node loads;
+//   * the remembered children are cached in a 16-entry per-lane LDS stack, so
+//     backtracking pops instead of walking parent links.  The trail stays the
+//     authority: when the cache has dropped an entry (deeper than 16 pending
+//     children), the walk falls back to the parent links.
+// Arithmetic is f32 with no FMA contraction and IEEE division, so the visit
+// order, hit triangle and distance are a pure function of the inputs (the
+// oracle reproduces them exactly).  The 128-bit trail limits trees to depth
+// 128, as in the reference.
 //
 // Rays are generated in f64 from a camera basis computed in f64 on the host
 // (the exact inverse of lookAtRH), then rounded once to f32.
@@ -56,27 +68,6 @@ __global__ __launch_bounds__(256) void k_genRays(bvht_Ray *__restrict__ rays, ui
   rays[(size_t)y * w + x] = r;
 }
 
-__device__ __forceinline__ float boxCoord(const bvht_AABB &b, uint32_t side, int axis) {
-  const bvht_vec3 &v = b.minMax[side];
-  return axis == 0 ? v.x : (axis == 1 ? v.y : v.z);
-}
-
-// intersectRayAABB -- CUDAKernels.cu:66-85
-__device__ __forceinline__ bool hitBox(const bvht_AABB &b, const bvht_Ray &r, float t0, float t1, float &tmin) {
-  float tmax, tymin, tymax, tzmin, tzmax;
-  tmin = (boxCoord(b, r.sign[0], 0) - r.pos.x) * r.invDir.x;
-  tmax = (boxCoord(b, 1 - r.sign[0], 0) - r.pos.x) * r.invDir.x;
-  tymin = (boxCoord(b, r.sign[1], 1) - r.pos.y) * r.invDir.y;
-  tymax = (boxCoord(b, 1 - r.sign[1], 1) - r.pos.y) * r.invDir.y;
-  tzmin = (boxCoord(b, r.sign[2], 2) - r.pos.z) * r.invDir.z;
-  tzmax = (boxCoord(b, 1 - r.sign[2], 2) - r.pos.z) * r.invDir.z;
-  if (tymin > tmin) tmin = tymin;
-  if (tzmin > tmin) tmin = tzmin;
-  if (tymax < tmax) tmax = tymax;
-  if (tzmax < tmax) tmax = tzmax;
-  return (tmin < tmax) && (tmin < t1) && (tmax > t0);
-}
-
 __device__ __forceinline__ bvht_vec3 vsub(bvht_vec3 a, bvht_vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ bvht_vec3 vcross(bvht_vec3 x, bvht_vec3 y) {
   return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
@@ -90,11 +81,101 @@ __device__ __forceinline__ bvht_vec3 vnormalize(bvht_vec3 v) {
   return {v.x * s, v.y * s, v.z * s};
 }
 
-// intersectRayTriangle2 -- CUDAKernels.cu:88-112, one-sided (the
-// cudaFaceCulling default, meson_options.txt:1).  IEEE 1/det replaces
-// __fdividef.
-__device__ __forceinline__ bool hitTri(const bvht_Ray &r, bvht_vec3 v0, bvht_vec3 v1, bvht_vec3 v2, float &dist) {
+struct TraceOut {
+  uint32_t tri;
+  float t;
+  uint16_t intCount;
+};
+
+// One inner node's children: boxes (min.xyz, max.xyz each), node ids and
+// leaf info (kInnerChild, the first face of a one-triangle leaf, or
+// kLookupLeaf for a leaf holding several triangles).
+struct PackedPair {
+  float4 a;  // l.min.x l.min.y l.min.z l.max.x
+  float4 b;  // l.max.y l.max.z r.min.x r.min.y
+  float4 c;  // r.min.z r.max.x r.max.y r.max.z
+  uint4 d;   // l r lInfo rInfo
+};
+struct PackedTri {
+  float4 v0, e1, e2;  // e1 = v1 - v0, e2 = v2 - v0 (w unused)
+};
+constexpr uint32_t kInnerChild = 0xFFFFFFFFu, kLookupLeaf = 0xFFFFFFFEu, kEmptyLeaf = 0xFFFFFFFDu;
+constexpr int kStackCap = 16;  // cached remembered children per lane (power of two)
+
+__device__ __forceinline__ uint32_t leafInfo(const bvht_BVHNode &n) {
+  if (n.isLeafFlag == 0) return kInnerChild;
+  return n.right == 1 ? n.left : kLookupLeaf;
+}
+
+// Packs every inner node below `bound`; a child index at or past `bound`
+// becomes an empty leaf and raises *bad (the traversal then never reads past
+// the arrays).
+__global__ __launch_bounds__(256) void k_packPairs(const bvht_BVHNode *__restrict__ nodes, uint32_t bound,
+                                                   PackedPair *__restrict__ pairs, uint32_t *__restrict__ bad) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= bound) return;
+  const bvht_BVHNode n = nodes[i];
+  if (n.isLeafFlag != 0) return;
+  uint32_t l = n.left, r = n.right, li = kEmptyLeaf, ri = kEmptyLeaf;
+  bvht_AABB lb = {{{0.f, 0.f, 0.f}, {-1.f, -1.f, -1.f}}}, rb = lb;
+  if (l < bound) {
+    const bvht_BVHNode c = nodes[l];
+    lb = c.bbox;
+    li = leafInfo(c);
+  } else {
+    *bad = 1;
+    l = i;
+  }
+  if (r < bound) {
+    const bvht_BVHNode c = nodes[r];
+    rb = c.bbox;
+    ri = leafInfo(c);
+  } else {
+    *bad = 1;
+    r = i;
+  }
+  PackedPair p;
+  p.a = make_float4(lb.minMax[0].x, lb.minMax[0].y, lb.minMax[0].z, lb.minMax[1].x);
+  p.b = make_float4(lb.minMax[1].y, lb.minMax[1].z, rb.minMax[0].x, rb.minMax[0].y);
+  p.c = make_float4(rb.minMax[0].z, rb.minMax[1].x, rb.minMax[1].y, rb.minMax[1].z);
+  p.d = make_uint4(l, r, li, ri);
+  pairs[i] = p;
+}
+
+__global__ __launch_bounds__(256) void k_packTris(const bvht_Triangle *__restrict__ faces,
+                                                  const bvht_vec3 *__restrict__ vert, uint32_t nf,
+                                                  PackedTri *__restrict__ tris) {
+  const uint32_t f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= nf) return;
+  const bvht_Triangle t = faces[f];
+  const bvht_vec3 v0 = vert[t.v1], v1 = vert[t.v2], v2 = vert[t.v3];
   const bvht_vec3 e1 = vsub(v1, v0), e2 = vsub(v2, v0);
+  tris[f] = PackedTri{make_float4(v0.x, v0.y, v0.z, 0.f), make_float4(e1.x, e1.y, e1.z, 0.f),
+                      make_float4(e2.x, e2.y, e2.z, 0.f)};
+}
+
+// intersectRayAABB -- CUDAKernels.cu:66-85, on a packed box.
+__device__ __forceinline__ bool hitBox6(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                        const bvht_Ray &r, float t0, float t1, float &tmin) {
+  float tmax, tymin, tymax, tzmin, tzmax;
+  tmin = ((r.sign[0] ? mxx : mnx) - r.pos.x) * r.invDir.x;
+  tmax = ((r.sign[0] ? mnx : mxx) - r.pos.x) * r.invDir.x;
+  tymin = ((r.sign[1] ? mxy : mny) - r.pos.y) * r.invDir.y;
+  tymax = ((r.sign[1] ? mny : mxy) - r.pos.y) * r.invDir.y;
+  tzmin = ((r.sign[2] ? mxz : mnz) - r.pos.z) * r.invDir.z;
+  tzmax = ((r.sign[2] ? mnz : mxz) - r.pos.z) * r.invDir.z;
+  if (tymin > tmin) tmin = tymin;
+  if (tzmin > tmin) tmin = tzmin;
+  if (tymax < tmax) tmax = tymax;
+  if (tzmax < tmax) tmax = tzmax;
+  return (tmin < tmax) && (tmin < t1) && (tmax > t0);
+}
+
+// intersectRayTriangle2 -- CUDAKernels.cu:88-112, one-sided (the
+// cudaFaceCulling default, meson_options.txt:1), with the edges precomputed
+// by k_packTris.  IEEE 1/det replaces __fdividef.
+__device__ __forceinline__ bool hitTriPacked(const bvht_Ray &r, const PackedTri &t, float &dist) {
+  const bvht_vec3 v0 = {t.v0.x, t.v0.y, t.v0.z}, e1 = {t.e1.x, t.e1.y, t.e1.z}, e2 = {t.e2.x, t.e2.y, t.e2.z};
   const bvht_vec3 p = vcross(r.dir, e2);
   const float det = vdot(e1, p);
   const bvht_vec3 tv = vsub(r.pos, v0);
@@ -106,49 +187,63 @@ __device__ __forceinline__ bool hitTri(const bvht_Ray &r, bvht_vec3 v0, bvht_vec
   return (det > 0.0f) && ((u >= 0.0f && u <= det) && (v >= 0.0f && (u + v) <= det));
 }
 
-struct TraceOut {
-  uint32_t tri;
-  float t;
-  uint16_t intCount;
+struct TraceCtx {
+  const PackedPair *pairs;
+  const PackedTri *tris;
+  const bvht_BVHNode *nodes;  // parent links (fallback) and multi-triangle leaves
+  uint32_t bound;             // node count the packing covered
+  uint32_t numFaces;
 };
 
-// kTraceRay traversal loop -- CUDAKernels.cu:146-204.
+// kTraceRay traversal loop -- CUDAKernels.cu:146-204, over packed nodes.
 // MODE 0: reference triangle test.  MODE 1: hits need t > tMin and are not
-// the triangle `skip` (secondary rays leaving a surface).
+// the triangle `skip` (secondary rays leaving a surface).  `stk` is this
+// lane's column of a [kStackCap][64] LDS array.
 template <int MODE>
-__device__ TraceOut traverse(const bvht_Ray &ray, const bvht_BVHNode *__restrict__ nodes, uint32_t root,
-                             const bvht_vec3 *__restrict__ vert, const bvht_Triangle *__restrict__ faces, float tMin,
-                             uint32_t skip) {
+__device__ TraceOut traversePacked(const bvht_Ray &ray, const TraceCtx &cx, uint32_t root, float tMin, uint32_t skip,
+                                   uint2 *stk) {
   TraceOut o = {kNoHit, HUGE_VALF, 0};
+  if (root >= cx.bound) return o;
   uint64_t lo = 0, hi = 0;
-  uint32_t node = root;
-  float nearest = HUGE_VALF, dist, minL, minR;
+  uint32_t node = root, info = leafInfo(cx.nodes[root]);
+  uint32_t top = 0, cached = 0;
+  float nearest = HUGE_VALF, dist;
   // A valid tree is walked in at most ~4 visits per node; the bound only
   // ends the loop on a malformed (cyclic) tree.
   for (uint32_t steps = 0; steps < (1u << 26); ++steps) {
-    const bvht_BVHNode *nd = nodes + node;
-    if (nd->isLeafFlag == 0) {
+    if (info == kInnerChild) {
       o.intCount++;
-      const uint32_t l = nd->left, r = nd->right;
-      const bool hl = hitBox(nodes[l].bbox, ray, 0.01f, nearest + 0.01f, minL);
-      const bool hr = hitBox(nodes[r].bbox, ray, 0.01f, nearest + 0.01f, minR);
+      const PackedPair pp = cx.pairs[node];
+      float minL, minR;
+      const bool hl = hitBox6(pp.a.x, pp.a.y, pp.a.z, pp.a.w, pp.b.x, pp.b.y, ray, 0.01f, nearest + 0.01f, minL);
+      const bool hr = hitBox6(pp.b.z, pp.b.w, pp.c.x, pp.c.y, pp.c.z, pp.c.w, ray, 0.01f, nearest + 0.01f, minR);
       if (hl || hr) {
         hi = (hi << 1) | (lo >> 63);
         lo <<= 1;
         if (hl && hr) {
           lo |= 1;
-          node = minL < minR ? l : r;
+          const bool leftFirst = minL < minR;
+          stk[top * 64] = leftFirst ? make_uint2(pp.d.y, pp.d.w) : make_uint2(pp.d.x, pp.d.z);
+          top = (top + 1) & (kStackCap - 1);
+          cached = cached < kStackCap ? cached + 1 : kStackCap;
+          node = leftFirst ? pp.d.x : pp.d.y;
+          info = leftFirst ? pp.d.z : pp.d.w;
         } else {
-          node = hl ? l : r;
+          node = hl ? pp.d.x : pp.d.y;
+          info = hl ? pp.d.z : pp.d.w;
         }
         continue;
       }
-    } else {
-      const uint32_t first = nd->left, count = nd->right;
+    } else if (info != kEmptyLeaf) {
+      uint32_t first = info, count = 1;
+      if (info == kLookupLeaf) {
+        first = cx.nodes[node].left;
+        count = cx.nodes[node].right;
+      }
       for (uint32_t k = 0; k < count; ++k) {
         const uint32_t t = first + k;
-        const bvht_Triangle tri = faces[t];
-        bool h = hitTri(ray, vert[tri.v1], vert[tri.v2], vert[tri.v3], dist);
+        if (t >= cx.numFaces) break;
+        bool h = hitTriPacked(ray, cx.tris[t], dist);
         if (MODE == 1) h = h && dist > tMin && t != skip;
         if (h && dist < nearest) {
           nearest = dist;
@@ -156,20 +251,36 @@ __device__ TraceOut traverse(const bvht_Ray &ray, const bvht_BVHNode *__restrict
         }
       }
     }
-    bool end = false;
-    while ((lo & 1) == 0) {
-      if (lo == 0 && hi == 0) {
-        end = true;
-        break;
+    // backtrack to the deepest remembered child
+    if (lo == 0 && hi == 0) break;
+    uint32_t tz;
+    if (lo != 0) {
+      tz = (uint32_t)__builtin_ctzll(lo);
+      if (tz) {
+        lo = (lo >> tz) | (hi << (64 - tz));
+        hi >>= tz;
       }
-      node = nodes[node].parent;
-      lo = (lo >> 1) | (hi << 63);
-      hi >>= 1;
+    } else {
+      tz = 64 + (uint32_t)__builtin_ctzll(hi);
+      lo = hi >> (tz - 64);
+      hi = 0;
     }
-    if (end) break;
-    const uint32_t p = nodes[node].parent;
-    node = nodes[node].isLeft == 0 ? nodes[p].left : nodes[p].right;
     lo ^= 1;
+    if (cached > 0) {
+      top = (top - 1) & (kStackCap - 1);
+      --cached;
+      const uint2 e = stk[top * 64];
+      node = e.x;
+      info = e.y;
+    } else {  // the entry fell out of the cache: walk the parent links
+      for (uint32_t k = 0; k < tz && node < cx.bound; ++k) node = cx.nodes[node].parent;
+      if (node >= cx.bound) break;
+      const uint32_t p = cx.nodes[node].parent;
+      if (p >= cx.bound) break;
+      node = cx.nodes[node].isLeft == 0 ? cx.nodes[p].left : cx.nodes[p].right;
+      if (node >= cx.bound) break;
+      info = leafInfo(cx.nodes[node]);
+    }
   }
   o.t = nearest;
   return o;
@@ -195,32 +306,32 @@ __device__ __forceinline__ bvht_CUDAPixel shade(const TraceOut &o, const bvht_Ra
 
 // Image path: 8x8 pixel tiles per wave (kTraceRay launch shape, CUDAKernels.cu:409-416).
 __global__ __launch_bounds__(64) void k_traceImage(const bvht_Ray *__restrict__ rays, bvht_CUDAPixel *__restrict__ img,
-                                                   const bvht_BVHNode *__restrict__ nodes, uint32_t root,
-                                                   const bvht_vec3 *__restrict__ vert,
+                                                   TraceCtx cx, uint32_t root, const bvht_vec3 *__restrict__ vert,
                                                    const bvht_Triangle *__restrict__ faces, bvht_vec3 light, uint32_t w,
                                                    uint32_t h) {
+  __shared__ uint2 s_stk[kStackCap * 64];
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   if (x >= w || y >= h) return;
   const size_t i = (size_t)y * w + x;
   const bvht_Ray ray = rays[i];
-  const TraceOut o = traverse<0>(ray, nodes, root, vert, faces, 0.0f, kNoHit);
+  const TraceOut o = traversePacked<0>(ray, cx, root, 0.0f, kNoHit, s_stk + threadIdx.x);
   img[i] = shade(o, ray, vert, faces, light);
 }
 
 template <int MODE>
-__global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ rays, uint32_t num,
-                                                  const bvht_BVHNode *__restrict__ nodes, uint32_t root,
-                                                  const bvht_vec3 *__restrict__ vert,
+__global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ rays, uint32_t num, TraceCtx cx,
+                                                  uint32_t root, const bvht_vec3 *__restrict__ vert,
                                                   const bvht_Triangle *__restrict__ faces, bvht_vec3 light, float tMin,
                                                   const uint32_t *__restrict__ skip, bvht_Hit *__restrict__ hits,
                                                   bvht_CUDAPixel *__restrict__ img) {
+  __shared__ uint2 s_stk[kStackCap * 64];
   const uint32_t i = blockIdx.x * 64 + threadIdx.x;
   if (i >= num) return;
   const bvht_Ray ray = rays[i];
   TraceOut o = {kNoHit, HUGE_VALF, 0};
   if (!(ray.dir.x == 0.0f && ray.dir.y == 0.0f && ray.dir.z == 0.0f))
-    o = traverse<MODE>(ray, nodes, root, vert, faces, tMin, skip ? skip[i] : kNoHit);
+    o = traversePacked<MODE>(ray, cx, root, tMin, skip ? skip[i] : kNoHit, s_stk + threadIdx.x);
   if (hits) hits[i] = bvht_Hit{o.tri, o.t};
   if (img) img[i] = shade(o, ray, vert, faces, light);
 }
@@ -277,6 +388,58 @@ __global__ __launch_bounds__(256) void k_bounce(const bvht_Ray *__restrict__ pri
   setRay(o, org, d, r.px, r.py);
   out[i] = o;
   skip[i] = hh.triId;
+}
+
+// Packed copies of the tree and mesh, rebuilt by every trace call (the tree
+// may have changed since the last one); the buffers only grow.
+struct TraceCache {
+  PackedPair *pairs = nullptr;
+  size_t pairCap = 0;
+  PackedTri *tris = nullptr;
+  size_t triCap = 0;
+  uint32_t *bad = nullptr;
+};
+TraceCache &traceCache() {
+  static TraceCache c;
+  return c;
+}
+
+// Node indices of the traced tree must lie below 2 * numFaces - 1 (every
+// binary BVH over the mesh fits); children past that are reported through
+// tracerDoCudaSync and traced as empty leaves.
+bool prepareTrace(const bvht_BVHNode *nodes, const bvht_MeshRaw &mesh, TraceCtx &cx, const char *what) {
+  if (!mesh.faces || !mesh.vert || mesh.numFaces == 0) {
+    setError("%s: empty mesh", what);
+    return false;
+  }
+  const uint64_t bound = 2ull * mesh.numFaces - 1;
+  if (bound >= kEmptyLeaf) {
+    setError("%s: %u faces exceed the 32-bit node range", what, mesh.numFaces);
+    return false;
+  }
+  TraceCache &c = traceCache();
+  if (c.pairCap < bound) {
+    devFree(c.pairs);
+    c.pairs = nullptr;
+    c.pairCap = 0;
+    if (!devAlloc(&c.pairs, bound, "trace node pairs")) return false;
+    c.pairCap = bound;
+  }
+  if (c.triCap < mesh.numFaces) {
+    devFree(c.tris);
+    c.tris = nullptr;
+    c.triCap = 0;
+    if (!devAlloc(&c.tris, mesh.numFaces, "trace triangles")) return false;
+    c.triCap = mesh.numFaces;
+  }
+  if (!c.bad && !devAlloc(&c.bad, 1, "trace flag")) return false;
+  if (!fillU32(c.bad, 0u, 1)) return false;
+  k_packPairs<<<divUp(bound, 256), 256, 0, stream()>>>(nodes, (uint32_t)bound, c.pairs, c.bad);
+  if (!checkLaunch("k_packPairs")) return false;
+  k_packTris<<<divUp(mesh.numFaces, 256), 256, 0, stream()>>>(mesh.faces, mesh.vert, mesh.numFaces, c.tris);
+  if (!checkLaunch("k_packTris")) return false;
+  cx = TraceCtx{c.pairs, c.tris, nodes, (uint32_t)bound, mesh.numFaces};
+  return true;
 }
 
 }  // namespace
@@ -337,8 +500,10 @@ void tracerImage(bvht_Ray *rays, bvht_CUDAPixel *img, bvht_BVHNode *nodes, uint3
     setError("tracerImage: missing rays, image or nodes");
     return;
   }
+  TraceCtx cx;
+  if (!prepareTrace(nodes, mesh, cx, "tracerImage")) return;
   dim3 grid(divUp(w, 8), divUp(h, 8));
-  k_traceImage<<<grid, 64, 0, stream()>>>(rays, img, nodes, rootNode, mesh.vert, mesh.faces, light, w, h);
+  k_traceImage<<<grid, 64, 0, stream()>>>(rays, img, cx, rootNode, mesh.vert, mesh.faces, light, w, h);
   checkLaunch("k_traceImage");
 }
 
@@ -348,7 +513,13 @@ void copyImageToHost(bvht_CUDAPixel *hostPixel, bvht_CUDAPixel *cudaImg, uint32_
     check(hipStreamSynchronize(stream()), "copyImageToHost sync");
 }
 
-void tracerDoCudaSync(void) { check(hipStreamSynchronize(stream()), "tracerDoCudaSync"); }
+void tracerDoCudaSync(void) {
+  if (!check(hipStreamSynchronize(stream()), "tracerDoCudaSync")) return;
+  uint32_t bad = 0;
+  TraceCache &c = traceCache();
+  if (c.bad && check(hipMemcpy(&bad, c.bad, 4, hipMemcpyDeviceToHost), "tracerDoCudaSync flag") && bad)
+    setError("tracer: a child index lies outside the 2 * numFaces - 1 nodes of the mesh");
+}
 
 bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
                     bvht_vec3 light, int mode, float tMin, const uint32_t *skipTri, bvht_Hit *hits,
@@ -358,11 +529,13 @@ bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uin
     return false;
   }
   if (num == 0) return true;
+  TraceCtx cx;
+  if (!prepareTrace(nodes, mesh, cx, "bvht_traceRays")) return false;
   if (mode == 0)
-    k_traceRays<0><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, nodes, rootNode, mesh.vert, mesh.faces, light, tMin,
+    k_traceRays<0><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
                                                        skipTri, hits, img);
   else
-    k_traceRays<1><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, nodes, rootNode, mesh.vert, mesh.faces, light, tMin,
+    k_traceRays<1><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
                                                        skipTri, hits, img);
   return checkLaunch("k_traceRays");
 }

@@ -521,13 +521,45 @@ class LBVHCmd : public Command {
   void teardown(State &) override { LBVH_freeWorkingMemory(&vMem); }
 };
 
+// Reads a little-endian fp32 .npy vector (format 1.x/2.x, C order).
+bool loadNpyF32(const std::string &path, std::vector<float> &out) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return false;
+  char magic[6];
+  uint8_t ver[2];
+  if (!in.read(magic, 6) || std::memcmp(magic, "\x93NUMPY", 6) != 0 || !in.read(reinterpret_cast<char *>(ver), 2))
+    return false;
+  uint32_t hlen = 0;
+  if (ver[0] == 1) {
+    uint16_t h16 = 0;
+    if (!in.read(reinterpret_cast<char *>(&h16), 2)) return false;
+    hlen = h16;
+  } else if (!in.read(reinterpret_cast<char *>(&hlen), 4)) {
+    return false;
+  }
+  std::string hdr(hlen, ' ');
+  if (!in.read(&hdr[0], hlen)) return false;
+  if (hdr.find("'<f4'") == std::string::npos || hdr.find("'fortran_order': False") == std::string::npos) return false;
+  std::vector<char> data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  if (data.size() % 4) return false;
+  out.resize(data.size() / 4);
+  std::memcpy(out.data(), data.data(), data.size());
+  return true;
+}
+
 // bittner13GPU -- lib/builder/Bittner13GPU.cpp:46-133 with the
-// OptimizerBase benchmark hooks (lib/builder/OptimizerBase.cpp:28-57)
+// OptimizerBase benchmark hooks (lib/builder/OptimizerBase.cpp:28-57).
+// Extension options: "priority" ("plain": the reference heuristic;
+// "learned": the MLP in "priorityModel", a .npy of bvht_mlpNumParams() fp32
+// values).  A learned step that does not lower the unit SAH hands the next
+// step to the plain heuristic.
 class Bittner13GPUCmd : public Command {
   uint32_t vMaxNumStepps = 500, vNumChunks = 32, vBlock = 128;
   float vBatchPercent = 1.0f;
   bool vSort = true, vOffsetAccess = false, vAltFindNode = false, vAltFixTree = false, vAltSort = false,
        vLocalPatchCPY = false, vCalcSAH = false, vDoSync = false;
+  std::string vPriority = "plain", vModelPath;
+  bvht_PriorityModel *vModel = nullptr;
   bvht_GPUWorkingMemory vMem = {};
   uint32_t chunkSize(const State &s) const {
     uint32_t n = (uint32_t)((vBatchPercent / 100.0f) * (float)s.dbvh.numNodes);
@@ -552,6 +584,8 @@ class Bittner13GPUCmd : public Command {
     vAltFixTree = j.value("altFixTree", vAltFixTree);
     vAltSort = j.value("altSort", vAltSort);
     vLocalPatchCPY = j.value("localPatchCPY", vLocalPatchCPY);
+    vPriority = j.value("priority", vPriority);
+    vModelPath = j.value("priorityModel", vModelPath);
     if (vBatchPercent <= 0.01f) vBatchPercent = 0.01f;
     if (vBatchPercent >= 75.0f) vBatchPercent = 75.0f;
   }
@@ -559,9 +593,14 @@ class Bittner13GPUCmd : public Command {
     return json{{"base_calcSAH", vCalcSAH}, {"base_doSync", vDoSync},     {"maxNumStepps", vMaxNumStepps},
                 {"numChunks", vNumChunks},  {"CUDABlockSize", vBlock},    {"batchPercent", vBatchPercent},
                 {"sort", vSort},            {"offsetAccess", vOffsetAccess}, {"altFindNode", vAltFindNode},
-                {"altFixTree", vAltFixTree}, {"altSort", vAltSort},       {"localPatchCPY", vLocalPatchCPY}};
+                {"altFixTree", vAltFixTree}, {"altSort", vAltSort},       {"localPatchCPY", vLocalPatchCPY},
+                {"priority", vPriority},    {"priorityModel", vModelPath}};
   }
   ErrorCode setup(State &s) override {
+    if (vPriority != "plain" && vPriority != "learned") {
+      LOG_ERR("bittner13GPU: priority must be \"plain\" or \"learned\", not \"%s\"", vPriority.c_str());
+      return ErrorCode::PARSE_ERROR;
+    }
     vMem = allocateMemory(&s.dbvh, chunkSize(s), s.rawMesh.numFaces);
     if (!vMem.result) {
       LOG_ERR("%s", libError().c_str());
@@ -571,6 +610,18 @@ class Bittner13GPUCmd : public Command {
     if (*bvht_getLastError()) {
       LOG_ERR("%s", libError().c_str());
       return ErrorCode::CUDA_ERROR;
+    }
+    if (vPriority == "learned") {
+      std::vector<float> params;
+      if (!loadNpyF32(vModelPath, params) || params.size() != bvht_mlpNumParams()) {
+        LOG_ERR("bittner13GPU: cannot read %u fp32 parameters from \"%s\"", bvht_mlpNumParams(), vModelPath.c_str());
+        return ErrorCode::IO_ERROR;
+      }
+      vModel = bvht_mlpCreate(0);
+      if (!vModel || !bvht_mlpSetParams(vModel, params.data()) || !bvht_bn13_setModel(&vMem, vModel)) {
+        LOG_ERR("%s", libError().c_str());
+        return ErrorCode::CUDA_ERROR;
+      }
     }
     return ErrorCode::OK;
   }
@@ -586,6 +637,9 @@ class Bittner13GPUCmd : public Command {
     c.altSort = vAltSort;
     c.sort = vSort;
     c.localPatchCPY = vLocalPatchCPY;
+    const bool learned = vPriority == "learned";
+    bool fallbackNext = false;
+    float prevSAH = learned ? CUDAcalcSAH(&s.dbvh) : 0.0f;
     s.numSkipped = 0;
     s.numTotal = 0;
     s.optSteps.push_back({0, vCalcSAH ? CUDAcalcSAH(&s.dbvh) : 0.0f, 0.0});
@@ -593,12 +647,18 @@ class Bittner13GPUCmd : public Command {
     for (uint32_t i = 0; i < vMaxNumStepps; ++i) {
       s.optSteps.push_back({(uint32_t)s.optSteps.size(), 0.0f, 0.0});
       if (vDoSync) doCudaDevSync();
+      c.priority = (learned && !fallbackNext) ? 1u : 0u;
       auto t0 = std::chrono::system_clock::now();
       bn13_doAlgorithmStep(&vMem, &s.dbvh, c);
       if (vDoSync) doCudaDevSync();
       auto t1 = std::chrono::system_clock::now();
       s.optSteps.back().ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-      if (vCalcSAH) s.optSteps.back().sah = CUDAcalcSAH(&s.dbvh);
+      if (vCalcSAH || learned) {
+        const float sah = CUDAcalcSAH(&s.dbvh);
+        if (vCalcSAH) s.optSteps.back().sah = sah;
+        fallbackNext = learned && c.priority == 1 && sah >= prevSAH;
+        prevSAH = sah;
+      }
       if (*bvht_getLastError()) {
         LOG_ERR("%s", libError().c_str());
         return ErrorCode::CUDA_ERROR;
@@ -613,6 +673,8 @@ class Bittner13GPUCmd : public Command {
     s.numTotal = k * vNumChunks * vMaxNumStepps;
     LOG_INFO("Skipped %u of %u", s.numSkipped, s.numTotal);
     freeMemory(&vMem);
+    bvht_mlpFree(vModel);
+    vModel = nullptr;
   }
 };
 

@@ -1,0 +1,164 @@
+"""Learned Bittner13 removal priority: records, training and evaluation.
+
+Records come from the plain GPU optimizer itself (1% of the nodes per step,
+32 chunks, the reference defaults) on seeded scenes of the C1 kind (icosphere
+clusters + triangle soup).  Every patch a step evaluates is recorded as the
+features of its node at selection time and the SAH gain the searches found for
+it.  Each step also probes uniformly sampled inner nodes, so the model sees
+nodes the heuristic would not have picked.  Targets are z-scored, and ranking is
+invariant to that.
+
+The model is trained with Adam (hand-written BF16-MFMA kernels, bvht_mlpTrain).
+Evaluation is time to reach the plain optimizer's final SAH(1.2, 1) within
+0.5% on held-out seeds. The learned run falls back to the plain heuristic for
+one step whenever a learned step fails to lower the true SAH.
+
+    python -m bvhtest_amd.priority_training --out weights.npy
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import (Bittner13GPU, BVHTestError, DeviceMesh, PriorityModel, calc_sah, lbvh_build, lib,
+               scene_spheres_soup)
+
+
+def scene(seed, spheres, soup):
+    return scene_spheres_soup(spheres, soup, seed)
+
+
+def generate_records(seeds, spheres, soup, steps, probes_per_step, batch_percent=1.0, chunks=32, log=None):
+    feats, gains = [], []
+    for seed in seeds:
+        v, f = scene(seed, spheres, soup)
+        mesh = DeviceMesh(v, f)
+        tree = lbvh_build(mesh, 63)
+        opt = Bittner13GPU(numChunks=chunks, batchPercent=batch_percent)
+        opt.setup(tree, len(f))
+        n_inner = opt.mem.numInnerNodes
+        per_step = opt.chunk * chunks + probes_per_step
+        opt.record(per_step * steps)
+        rng = np.random.default_rng(seed)
+        for _ in range(steps):
+            opt.probe(tree, rng.integers(0, n_inner, probes_per_step))
+            opt.step(tree)
+        lib().doCudaDevSync()
+        x, g = opt.records()
+        feats.append(x)
+        gains.append(g)
+        if log:
+            log(f"records: seed {seed}, {len(f)} triangles, {len(g)} records")
+        opt.free()
+        tree.free()
+    x = np.concatenate(feats)
+    g = np.concatenate(gains)
+    keep = np.isfinite(g) & np.all(np.isfinite(x), axis=1)
+    return x[keep], g[keep]
+
+
+def train(x, g, seed=0, epochs=(12, 6, 4), lrs=(2e-3, 5e-4, 1e-4), batch=4096, log=None):
+    mu, sd = float(g.mean()), float(g.std() + 1e-12)
+    t = ((g - mu) / sd).astype(np.float32)
+    model = PriorityModel(seed=seed)
+    for e, lr in zip(epochs, lrs):
+        loss = model.train(x, t, epochs=e, batch=batch, lr=lr, seed=seed + e)
+        if log:
+            log(f"train: {e} epochs at lr {lr}: loss {loss:.4f}")
+    pred = model.predict(x)
+    rank = np.corrcoef(np.argsort(np.argsort(pred)), np.argsort(np.argsort(t)))[0, 1]
+    return model, {"target_mean": mu, "target_std": sd, "train_rank_corr": float(rank), "records": int(len(t))}
+
+
+def _timed_step(opt, tree):
+    t0 = time.perf_counter()
+    opt.step(tree)
+    lib().doCudaDevSync()
+    return time.perf_counter() - t0
+
+
+def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=32, max_factor=2.0):
+    """Plain: `plain_steps` steps -> SAH S_p in time T_p.  Learned: steps until
+    SAH <= 1.005 S_p (at most max_factor * plain_steps) -> T_l.  Step times
+    include selection (features + inference for learned) and are measured
+    with a device sync per step; the SAH evaluations that judge each step
+    are not timed."""
+    v, f = scene(seed, spheres, soup)
+    mesh = DeviceMesh(v, f)
+    res = {"seed": seed, "triangles": int(len(f))}
+    plain_tree = lbvh_build(mesh, 63)
+    s0 = calc_sah(plain_tree, 1.2, 1.0)
+    opt = Bittner13GPU(numChunks=chunks, batchPercent=batch_percent)
+    opt.setup(plain_tree, len(f))
+    tp = sum(_timed_step(opt, plain_tree) for _ in range(plain_steps))
+    sp = calc_sah(plain_tree, 1.2, 1.0)
+    opt.free()
+    plain_tree.free()
+    res.update(sah_initial=s0, plain_steps=plain_steps, plain_s=tp, plain_sah=sp)
+
+    tree = lbvh_build(mesh, 63)
+    opt = Bittner13GPU(numChunks=chunks, batchPercent=batch_percent, priority=1)
+    opt.setup(tree, len(f))
+    opt.set_model(model)
+    target = 1.005 * sp
+    tl, prev, steps, fallbacks, fallback_next = 0.0, s0, 0, 0, False
+    sah = s0
+    while steps < int(max_factor * plain_steps):
+        opt.set_priority(0 if fallback_next else 1)
+        tl += _timed_step(opt, tree)
+        steps += 1
+        sah = calc_sah(tree, 1.2, 1.0)
+        if fallback_next:
+            fallbacks += 1
+            fallback_next = False
+        elif sah >= prev:
+            fallback_next = True  # the plain heuristic takes the next step
+        prev = sah
+        if sah <= target:
+            break
+    res.update(learned_steps=steps, learned_s=tl, learned_sah=sah, fallback_steps=fallbacks,
+               reached=bool(sah <= target), time_ratio=(tl / tp) if sah <= target else None)
+    opt.free()
+    tree.free()
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True, help="weights (.npy, fp32)")
+    ap.add_argument("--train-seeds", default="100-107")
+    ap.add_argument("--eval-seeds", default="200-201")
+    ap.add_argument("--spheres", type=int, default=512)
+    ap.add_argument("--soup", type=int, default=125000)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--probes", type=int, default=20000)
+    ap.add_argument("--eval-spheres", type=int, default=2048)
+    ap.add_argument("--eval-soup", type=int, default=500000)
+    ap.add_argument("--eval-steps", type=int, default=50)
+    args = ap.parse_args(argv)
+
+    def seeds(spec):
+        a, _, b = spec.partition("-")
+        return list(range(int(a), int(b or a) + 1))
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    x, g = generate_records(seeds(args.train_seeds), args.spheres, args.soup, args.steps, args.probes, log=log)
+    model, info = train(x, g, log=log)
+    model.save(args.out)
+    log(f"saved {args.out}: {json.dumps(info)}")
+    for s in seeds(args.eval_seeds):
+        r = evaluate(model, s, args.eval_spheres, args.eval_soup, args.eval_steps)
+        log(f"eval: {json.dumps(r)}")
+        print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    try:
+        main()
+    except BVHTestError as e:
+        print(f"error: {e}", file=sys.stderr)
+        sys.exit(1)

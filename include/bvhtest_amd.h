@@ -297,7 +297,7 @@ typedef struct bvht_AlgoCFG {
   bool localPatchCPY;
   /* extension */
   uint32_t priority;       /* 0: reference heuristic (kCalcCost), 1: learned MLP */
-  uint32_t lanesPerSearch; /* lanes cooperating on one insertion search (0 = default) */
+  uint32_t lanesPerSearch; /* reserved, 0 */
 } bvht_AlgoCFG;
 
 /* GPUWorkingMemory -- lib/builder/Bittner13CUDA.hpp:39-54 (opaque here). */
@@ -348,6 +348,42 @@ int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords);
  * sum over non-root nodes of SA * (leaf ? cLeaf : cInner) / SA(root)),
  * accumulated in f64 (extension). */
 double bvht_calcSAHWeighted(bvht_CUDAMemoryBVHPointer *bvh, float cInner, float cLeaf);
+
+/* ---- learned removal priority (extension; the reference ranks nodes by the
+ * fixed heuristic of Bittner13CUDA.cu:480-500).  A 12-64-64-1 MLP with ReLU,
+ * BF16 weights on MFMA and FP32 accumulation; the 12 node features are
+ * listed in bvhtest_amd/csrc/mlp.hip.  Parameters are kNumParams fp32 values:
+ * w1[12][64] (in, out), b1[64], w2[64][64] (in, out), b2[64], w3[64], b3. */
+typedef struct bvht_PriorityModel bvht_PriorityModel;
+uint32_t bvht_mlpNumParams(void);
+/* He-uniform weights from `seed`, zero biases; NULL on failure. */
+bvht_PriorityModel *bvht_mlpCreate(uint32_t seed);
+void bvht_mlpFree(bvht_PriorityModel *model);
+bool bvht_mlpSetParams(bvht_PriorityModel *model, const float *hostParams);
+bool bvht_mlpGetParams(bvht_PriorityModel *model, float *hostParams);
+/* devOut[i] = model(devFeat[i][0..11]) for i < num (device buffers). */
+bool bvht_mlpPredict(bvht_PriorityModel *model, const float *devFeat, uint32_t num, float *devOut);
+/* Adam on the mean squared error of model(devFeat[i]) against devTarget[i]:
+ * `epochs` passes over a seeded shuffle in batches of `batch` (partial last
+ * batch dropped).  Returns the mean loss of the last epoch, -1 on error. */
+float bvht_mlpTrain(bvht_PriorityModel *model, const float *devFeat, const float *devTarget, uint32_t num,
+                    uint32_t epochs, uint32_t batch, float lr, uint32_t seed);
+/* Model used by bn13_selectNodes when AlgoCFG.priority == 1 (NULL clears). */
+bool bvht_bn13_setModel(bvht_GPUWorkingMemory *data, bvht_PriorityModel *model);
+/* Features of every inner node, in inner-list order (ascending node id),
+ * into devFeat[numInnerNodes][12]. */
+bool bvht_bn13_features(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, float *devFeat);
+/* SAH-gain records: while enabled (maxRecords > 0), every patch a step
+ * evaluates appends (features of its node at selection time, gain), where
+ * gain = (SA(node) + SA(parent) + shrink of the two path boxes above the
+ * removal - both insertion costs) / SA(root).  Re-enabling clears. */
+bool bvht_bn13_setRecording(bvht_GPUWorkingMemory *data, uint32_t maxRecords);
+/* Number of records held (<= maxRecords); device pointers to them. */
+uint32_t bvht_bn13_records(bvht_GPUWorkingMemory *data, float **devFeat, float **devGain);
+/* Records the patches of the inner nodes at devPositions[0..num) (inner-list
+ * positions) on the current tree without changing it. */
+bool bvht_bn13_probe(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg,
+                     const uint32_t *devPositions, uint32_t num);
 
 /* ------------------------------------------------------------------------ */
 /* Ray tracer -- lib/tracer/CUDAKernels.hpp:39-78 (extern "C" in reference)  */
