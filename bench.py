@@ -47,12 +47,37 @@ def kernel_bytes(name, n, morton_bits, pass_idx, passes):
     return None
 
 
+def profile_build(args):
+    """Counter runs (rocprofv3 --pmc): the timed LBVH builds only, then a
+    device-to-device copy of 256 MiB, the calibration kernel whose counters
+    per byte scale the build's FETCH_SIZE / WRITE_SIZE to bytes."""
+    import bvhtest_amd as bt
+
+    L = bt.lib()
+    v, f = bt.scene_spheres_soup(2048, 500000, 2)
+    mesh = bt.DeviceMesh(v, f)
+    builder = bt.LBVHBuilder(mesh, args.morton_bits)
+    tree = builder.allocate_bvh()
+    for _ in range(args.steps):
+        builder.build(tree)
+    L.LBVH_doCUDASyc()
+    nbytes = 256 << 20
+    a, b = bt.DeviceBuffer(nbytes), bt.DeviceBuffer(nbytes)
+    for _ in range(3):
+        if not L.runMemcpy(b.ptr, a.ptr, nbytes, 3):
+            raise SystemExit("device copy failed")
+    L.LBVH_doCUDASyc()
+    print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes}), flush=True)
+
+
 def run_bench(args):
     import bvhtest_amd as bt
 
     L = bt.lib()
     if L.bvht_deviceCount() < 1:
         raise SystemExit("bench.py needs a HIP device")
+    if args.profile_build:
+        return profile_build(args)
     t0 = time.time()
     v, f = bt.scene_spheres_soup(2048, 500000, 2)
     n = len(f)
@@ -282,6 +307,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--profile-build", action="store_true", help="builds + a 256 MiB copy only (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "

@@ -316,11 +316,12 @@ def trace_rays(tree, mesh, rays, light=(2.0, 2.0, 2.0), mode=0, tmin=0.0, skip=N
     return pb.download(PIXEL_DTYPE, n), hb.download(HIT_DTYPE, n)
 
 
-def trace_image(tree, mesh, w, h, pos, look_at, up, fov, light=(2.0, 2.0, 2.0), root=0):
-    """generateRays + tracerImage + copyImageToHost: the CUDATracer command."""
+def trace_image(tree, mesh, w, h, pos, look_at, up, fov, light=(2.0, 2.0, 2.0), root=0, bundle=False):
+    """generateRays + tracerImage + copyImageToHost: the CUDATracer command
+    (bundle: the 8x8 ray-bundle traversal, option rayBundles)."""
     rays = generate_rays_device(w, h, pos, look_at, up, fov)
     img = DeviceBuffer(w * h * PIXEL_DTYPE.itemsize)
-    lib().tracerImage(rays.ptr, img.ptr, tree.ptr.nodes, root, mesh.raw, vec3(light), w, h, False)
+    lib().tracerImage(rays.ptr, img.ptr, tree.ptr.nodes, root, mesh.raw, vec3(light), w, h, bool(bundle))
     if last_error():
         _fail("tracerImage")
     out = np.empty(w * h, PIXEL_DTYPE)

@@ -319,6 +319,96 @@ __global__ __launch_bounds__(64) void k_traceImage(const bvht_Ray *__restrict__ 
   img[i] = shade(o, ray, vert, faces, light);
 }
 
+// Ray bundles (kTraceRayBundle, CUDAKernels.cu:240-383): the 64 rays of an
+// 8x8 tile -- one wave -- walk one shared node sequence.  At an inner node
+// each ray tests both children against its own nearest hit; the bundle
+// enters every child some ray hits, the one a majority prefers first (ties:
+// right), and backtracks on one shared trail.  Ballots replace the
+// reference's shared-memory flags and reduction.  Every ray still finds its
+// nearest hit (the shared walk covers its own); among triangles at exactly
+// the same distance the first one met may differ from the per-ray order.
+__global__ __launch_bounds__(64) void k_traceImageBundle(const bvht_Ray *__restrict__ rays,
+                                                         bvht_CUDAPixel *__restrict__ img, TraceCtx cx, uint32_t root,
+                                                         const bvht_vec3 *__restrict__ vert,
+                                                         const bvht_Triangle *__restrict__ faces, bvht_vec3 light,
+                                                         uint32_t w, uint32_t h) {
+  __shared__ uint2 s_stk[128];  // pending children: at most one per trail bit
+  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7);
+  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  const bool valid = x < w && y < h;
+  bvht_Ray ray;
+  if (valid) ray = rays[(size_t)y * w + x];
+  TraceOut o = {kNoHit, HUGE_VALF, 0};
+  if (root < cx.bound) {
+    uint64_t lo = 0, hi = 0;  // wave-uniform from here on
+    uint32_t node = root, info = leafInfo(cx.nodes[root]), sp = 0;
+    float nearest = HUGE_VALF, dist;
+    for (uint32_t steps = 0; steps < (1u << 26); ++steps) {
+      if (info == kInnerChild) {
+        o.intCount++;
+        const PackedPair pp = cx.pairs[node];
+        float minL = 0.0f, minR = 0.0f;
+        const bool hl =
+            valid && hitBox6(pp.a.x, pp.a.y, pp.a.z, pp.a.w, pp.b.x, pp.b.y, ray, 0.01f, nearest + 0.01f, minL);
+        const bool hr =
+            valid && hitBox6(pp.b.z, pp.b.w, pp.c.x, pp.c.y, pp.c.z, pp.c.w, ray, 0.01f, nearest + 0.01f, minR);
+        const bool anyL = __ballot(hl) != 0, anyR = __ballot(hr) != 0;
+        if (anyL || anyR) {
+          hi = (hi << 1) | (lo >> 63);
+          lo <<= 1;
+          if (anyL && anyR) {
+            lo |= 1;
+            const bool wantL = hl && (minL < minR || !hr), wantR = hr && (minR < minL || !hl);
+            const int vote = __popcll(__ballot(wantL)) - __popcll(__ballot(wantR));
+            const bool leftFirst = vote > 0;
+            if (sp < 128) s_stk[sp] = leftFirst ? make_uint2(pp.d.y, pp.d.w) : make_uint2(pp.d.x, pp.d.z);
+            ++sp;
+            node = leftFirst ? pp.d.x : pp.d.y;
+            info = leftFirst ? pp.d.z : pp.d.w;
+          } else {
+            node = anyL ? pp.d.x : pp.d.y;
+            info = anyL ? pp.d.z : pp.d.w;
+          }
+          continue;
+        }
+      } else if (info != kEmptyLeaf) {
+        uint32_t first = info, count = 1;
+        if (info == kLookupLeaf) {
+          first = cx.nodes[node].left;
+          count = cx.nodes[node].right;
+        }
+        for (uint32_t k = 0; k < count; ++k) {
+          const uint32_t t = first + k;
+          if (t >= cx.numFaces) break;
+          if (valid && hitTriPacked(ray, cx.tris[t], dist) && dist < nearest) {
+            nearest = dist;
+            o.tri = t;
+          }
+        }
+      }
+      if (lo == 0 && hi == 0) break;
+      if (lo != 0) {
+        const uint32_t tz = (uint32_t)__builtin_ctzll(lo);
+        if (tz) {
+          lo = (lo >> tz) | (hi << (64 - tz));
+          hi >>= tz;
+        }
+      } else {
+        const uint32_t tz = 64 + (uint32_t)__builtin_ctzll(hi);
+        lo = hi >> (tz - 64);
+        hi = 0;
+      }
+      lo ^= 1;
+      if (sp == 0 || sp > 128) break;  // a trail deeper than 128 levels (unsupported, as in the reference)
+      const uint2 e = s_stk[--sp];
+      node = e.x;
+      info = e.y;
+    }
+    o.t = nearest;
+  }
+  if (valid) img[(size_t)y * w + x] = shade(o, ray, vert, faces, light);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ rays, uint32_t num, TraceCtx cx,
                                                   uint32_t root, const bvht_vec3 *__restrict__ vert,
@@ -495,7 +585,6 @@ void generateRays(bvht_Ray *rays, uint32_t w, uint32_t h, bvht_vec3 pos, bvht_ve
 
 void tracerImage(bvht_Ray *rays, bvht_CUDAPixel *img, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
                  bvht_vec3 light, uint32_t w, uint32_t h, bool bundle) {
-  (void)bundle;  // the wave-coherent variant traces the same hits; see bvht_traceRays
   if (!rays || !img || !nodes) {
     setError("tracerImage: missing rays, image or nodes");
     return;
@@ -503,8 +592,11 @@ void tracerImage(bvht_Ray *rays, bvht_CUDAPixel *img, bvht_BVHNode *nodes, uint3
   TraceCtx cx;
   if (!prepareTrace(nodes, mesh, cx, "tracerImage")) return;
   dim3 grid(divUp(w, 8), divUp(h, 8));
-  k_traceImage<<<grid, 64, 0, stream()>>>(rays, img, cx, rootNode, mesh.vert, mesh.faces, light, w, h);
-  checkLaunch("k_traceImage");
+  if (bundle)
+    k_traceImageBundle<<<grid, 64, 0, stream()>>>(rays, img, cx, rootNode, mesh.vert, mesh.faces, light, w, h);
+  else
+    k_traceImage<<<grid, 64, 0, stream()>>>(rays, img, cx, rootNode, mesh.vert, mesh.faces, light, w, h);
+  checkLaunch(bundle ? "k_traceImageBundle" : "k_traceImage");
 }
 
 void copyImageToHost(bvht_CUDAPixel *hostPixel, bvht_CUDAPixel *cudaImg, uint32_t w, uint32_t h) {

@@ -61,6 +61,21 @@ def test_trace_image_command(bvh, orc, c0):
     assert img.tobytes() == opix.tobytes()
 
 
+@pytest.mark.parametrize("scene", ["c0", "spheres"])
+def test_ray_bundles_match_single_rays(bvh, orc, scene, request):
+    """rayBundles (kTraceRayBundle semantics): the shared walk covers every
+    ray's own, so the hit mask is identical; the diffuse value may differ
+    only where two triangles lie at exactly the same distance."""
+    v, f, mesh, tree, nodes = request.getfixturevalue(scene)
+    cam = C0_CAM if scene == "c0" else BIG_CAM
+    w, h = 517, 301  # partial 8x8 tiles at the borders
+    single = bvh.trace_image(tree, mesh, w, h, **cam)
+    bundle = bvh.trace_image(tree, mesh, w, h, bundle=True, **cam)
+    assert np.array_equal(single["hit"], bundle["hit"])
+    assert single["hit"].sum() > 0.05 * w * h
+    assert (single["diffuse"] != bundle["diffuse"]).mean() <= 1e-3
+
+
 def test_trace_spheres_bit_exact(bvh, orc, spheres):
     v, f, mesh, tree, nodes = spheres
     rays = bvh.generate_rays(384, 216, **BIG_CAM)
