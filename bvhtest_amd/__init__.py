@@ -473,6 +473,15 @@ def calc_sah(tree, c_inner=1.0, c_leaf=1.0):
     return lib().bvht_calcSAHWeighted(ctypes.byref(tree.ptr), c_inner, c_leaf)
 
 
+def validate_device(tree, mesh, root=0):
+    """Device-side Validate: (total, [tree, bbox, strict, SA, triangle, orphan] counts)."""
+    err = (ctypes.c_uint32 * 6)()
+    total = lib().bvht_validate(ctypes.byref(tree.ptr), root, mesh.raw, err)
+    if total == 0xFFFFFFFF:
+        _fail("bvht_validate")
+    return total, list(err)
+
+
 # ---------------------------------------------------------------------------
 # learned removal priority (12-64-64-1 MLP, BF16 MFMA)
 # ---------------------------------------------------------------------------

@@ -348,6 +348,12 @@ int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords);
  * sum over non-root nodes of SA * (leaf ? cLeaf : cInner) / SA(root)),
  * accumulated in f64 (extension). */
 double bvht_calcSAHWeighted(bvht_CUDAMemoryBVHPointer *bvh, float cInner, float cLeaf);
+/* Validate on the device (lib/misc/Validate.cpp:42-263 without the level
+ * checks, which need the host-side fixLevels): errOut[6] receives the counts
+ * of tree, bbox, strict-bbox, surface-area, triangle and orphan-triangle
+ * errors (the classes are defined in bvhtest_amd/csrc/validate.hip).
+ * Returns their sum, or UINT32_MAX on a launch or argument error. */
+uint32_t bvht_validate(const bvht_CUDAMemoryBVHPointer *bvh, uint32_t root, bvht_MeshRaw mesh, uint32_t *errOut);
 
 /* ---- learned removal priority (extension; the reference ranks nodes by the
  * fixed heuristic of Bittner13CUDA.cu:480-500).  A 12-64-64-1 MLP with ReLU,
