@@ -133,6 +133,9 @@ SIGNATURES = {
     "bvht_calcSAHWeighted": (ctypes.c_double, [P(CUDAMemoryBVHPointer), c_f, c_f]),
     "bvht_bn13_stats": (ctypes.c_int, [P(GPUWorkingMemory), P(c_u32), ctypes.c_int]),
     "bvht_validate": (c_u32, [P(CUDAMemoryBVHPointer), c_u32, MeshRaw, P(c_u32)]),
+    "selectDynamicLeafNodes": (None, [P(GPUWorkingMemory), P(CUDAMemoryBVHPointer), ctypes.c_void_p, P(c_u32), c_u32]),
+    "refitDynamicTris": (None, [P(CUDAMemoryBVHPointer), MeshRaw, ctypes.c_void_p, c_u32, c_u32]),
+    "bvht_fixTreeFrom": (ctypes.c_bool, [P(GPUWorkingMemory), P(CUDAMemoryBVHPointer), ctypes.c_void_p, c_u32]),
     "bvht_mlpNumParams": (c_u32, []),
     "bvht_mlpCreate": (ctypes.c_void_p, [c_u32]),
     "bvht_mlpFree": (None, [ctypes.c_void_p]),

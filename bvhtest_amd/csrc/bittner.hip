@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_refitCount(const uint32_t *__restrict__
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= num) return;
   uint32_t node = starts[i];
-  if (node == NONE) return;
+  if (node >= N) return;
   for (uint32_t steps = 0; steps < N; ++steps) {
     if (atomicAdd(cnt + node, 1u) != 0) return;
     const uint32_t p = nodes[node].parent;
@@ -651,7 +651,13 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= num) return;
   uint32_t node = starts[i];
-  if (node == NONE) return;
+  if (node >= N) return;
+  if (nodes[node].isLeafFlag != 0) {  // a leaf start (dynamic refit) is counted once and not recomputed
+    __hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t p = nodes[node].parent;
+    if (p == node || p >= N) return;
+    node = p;
+  }
   for (uint32_t steps = 0; steps < N; ++steps) {
     // release of the boxes written on the previous trip: fence after the
     // stores, then a relaxed count (see fenceReleaseAgent)
@@ -756,6 +762,35 @@ __global__ __launch_bounds__(256) void k_sahPartials(const bvht_BVHNode *__restr
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+// selectDynamicLeafNodes (CUDAKernels.cu:456-484): leaves whose first face
+// is at or after `begin`, in leaf-list order (per-block counts, host scan,
+// ordered write -- the order DeviceSelect::If keeps).
+__global__ __launch_bounds__(kSelBlock) void k_dynCount(const bvht_BVHNode *__restrict__ nodes,
+                                                        const uint32_t *__restrict__ leaves, uint32_t num,
+                                                        uint32_t begin, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_scan[kSelBlock / 64];
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kSelTile + threadIdx.x * kSelItems;
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < num && nodes[leaves[base + j]].left >= begin) c++;
+  const uint32_t ex = blockExclusiveScan<kSelBlock>(c, s_scan);
+  if (threadIdx.x == kSelBlock - 1) counts[blockIdx.x] = ex + c;
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_dynWrite(const bvht_BVHNode *__restrict__ nodes,
+                                                        const uint32_t *__restrict__ leaves, uint32_t num,
+                                                        uint32_t begin, const uint32_t *__restrict__ offsets,
+                                                        uint32_t *__restrict__ out) {
+  __shared__ uint32_t s_scan[kSelBlock / 64];
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kSelTile + threadIdx.x * kSelItems;
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < num && nodes[leaves[base + j]].left >= begin) c++;
+  uint32_t pos = offsets[blockIdx.x] + blockExclusiveScan<kSelBlock>(c, s_scan);
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < num && nodes[leaves[base + j]].left >= begin) out[pos++] = leaves[base + j];
 }
 
 struct Internal {
@@ -999,6 +1034,57 @@ void fixTree3(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint3
   if (!checkLaunch("k_refitCount")) return;
   k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
   checkLaunch("k_refitApply");
+}
+
+void selectDynamicLeafNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t *out,
+                            uint32_t *numOut, uint32_t beginDynamicTriangle) {
+  Internal *in = inOf(data);
+  if (numOut) *numOut = 0;
+  if (!in || !validBVH(bvh, "selectDynamicLeafNodes") || !out || !numOut) {
+    if (!in) setError("selectDynamicLeafNodes: working memory not allocated");
+    else if (!out || !numOut) setError("selectDynamicLeafNodes: no output");
+    return;
+  }
+  const uint32_t blocks = divUp(in->numLeaves, kSelTile);
+  k_dynCount<<<blocks, kSelBlock, 0, stream()>>>(bvh->nodes, in->leaves, in->numLeaves, beginDynamicTriangle,
+                                                 in->selCounts);
+  if (!checkLaunch("k_dynCount")) return;
+  std::vector<uint32_t> counts(blocks);
+  if (!check(hipMemcpyAsync(counts.data(), in->selCounts, 4ull * blocks, hipMemcpyDeviceToHost, stream()),
+             "selectDynamicLeafNodes counts") ||
+      !check(hipStreamSynchronize(stream()), "selectDynamicLeafNodes sync"))
+    return;
+  uint32_t total = 0;
+  for (auto &c : counts) {
+    const uint32_t t = c;
+    c = total;
+    total += t;
+  }
+  if (!check(hipMemcpyAsync(in->selCounts, counts.data(), 4ull * blocks, hipMemcpyHostToDevice, stream()),
+             "selectDynamicLeafNodes offsets"))
+    return;
+  k_dynWrite<<<blocks, kSelBlock, 0, stream()>>>(bvh->nodes, in->leaves, in->numLeaves, beginDynamicTriangle,
+                                                 in->selCounts, out);
+  if (!checkLaunch("k_dynWrite") || !check(hipStreamSynchronize(stream()), "selectDynamicLeafNodes sync")) return;
+  *numOut = total;
+}
+
+bool bvht_fixTreeFrom(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, const uint32_t *devStarts,
+                      uint32_t num) {
+  Internal *in = inOf(data);
+  if (!in || !validBVH(bvh, "bvht_fixTreeFrom")) {
+    if (!in) setError("bvht_fixTreeFrom: working memory not allocated");
+    return false;
+  }
+  if (num == 0) return true;
+  if (!devStarts) {
+    setError("bvht_fixTreeFrom: no start nodes");
+    return false;
+  }
+  k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(devStarts, num, bvh->nodes, in->N, in->cnt);
+  if (!checkLaunch("k_refitCount")) return false;
+  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(devStarts, num, bvh->nodes, in->N, in->cnt);
+  return checkLaunch("k_refitApply");
 }
 
 void bn13_selectNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg) {

@@ -23,6 +23,7 @@
 // Rays are generated in f64 from a camera basis computed in f64 on the host
 // (the exact inverse of lookAtRH), then rounded once to f32.
 #include "runtime.hpp"
+#include "types.hpp"
 
 #include <cmath>
 
@@ -426,6 +427,24 @@ __global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ r
   if (img) img[i] = shade(o, ray, vert, faces, light);
 }
 
+// kRefitLeafBVHTris (CUDAKernels.cu:486-520): leaf boxes from the current
+// vertices (triangle box -+ FLT_EPSILON).  The leaf's SA is recomputed too
+// (the reference leaves it stale).
+__global__ __launch_bounds__(256) void k_refitLeafTris(const uint32_t *__restrict__ ids, uint32_t num,
+                                                       bvht_BVHNode *__restrict__ nodes,
+                                                       const bvht_Triangle *__restrict__ faces,
+                                                       const bvht_vec3 *__restrict__ vert, uint32_t numFaces) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= num) return;
+  const uint32_t id = ids[i];
+  const uint32_t f = nodes[id].left;
+  if (f >= numFaces) return;
+  const bvht_Triangle t = faces[f];
+  const bvht_AABB b = triangleBox(vert[t.v1], vert[t.v2], vert[t.v3]);
+  nodes[id].bbox = b;
+  nodes[id].surfaceArea = surfaceArea(b);
+}
+
 // Counter-based hash RNG (splitmix64 finaliser) -> uniform floats in [0, 1).
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
@@ -630,6 +649,19 @@ bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uin
     k_traceRays<1><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
                                                        skipTri, hits, img);
   return checkLaunch("k_traceRays");
+}
+
+void refitDynamicTris(bvht_CUDAMemoryBVHPointer *bvh, bvht_MeshRaw mesh, uint32_t *nodesToUpdate, uint32_t numNodes,
+                      uint32_t blockSize) {
+  (void)blockSize;
+  if (!bvh || !bvh->nodes || (numNodes && !nodesToUpdate) || !mesh.faces || !mesh.vert) {
+    setError("refitDynamicTris: missing tree, mesh or node list");
+    return;
+  }
+  if (numNodes == 0) return;
+  k_refitLeafTris<<<divUp(numNodes, 256), 256, 0, stream()>>>(nodesToUpdate, numNodes, bvh->nodes, mesh.faces,
+                                                              mesh.vert, mesh.numFaces);
+  checkLaunch("k_refitLeafTris");
 }
 
 bool bvht_makeBounceRays(const bvht_Ray *primary, const bvht_Hit *hits, uint32_t num, bvht_MeshRaw mesh, uint32_t seed,

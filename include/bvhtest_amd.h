@@ -329,6 +329,22 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
                           uint32_t chunk);
 /* bn13_doAlgorithmStep -- lib/builder/Bittner13CUDA.cu:768-772 */
 void bn13_doAlgorithmStep(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, bvht_AlgoCFG cfg);
+/* selectDynamicLeafNodes -- lib/tracer/CUDAKernels.cu:456-484: the leaves
+ * (device ids, leaf-list order) whose first face is >= beginDynamicTriangle
+ * into out[]; their number into *numOut (host). */
+void selectDynamicLeafNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t *out,
+                            uint32_t *numOut, uint32_t beginDynamicTriangle);
+/* refitDynamicTris -- lib/tracer/CUDAKernels.cu:486-530: leaf boxes of
+ * nodesToUpdate[0..numNodes) from the current vertices (the leaf SA is
+ * recomputed as well; the reference leaves it stale). */
+void refitDynamicTris(bvht_CUDAMemoryBVHPointer *bvh, bvht_MeshRaw mesh, uint32_t *nodesToUpdate, uint32_t numNodes,
+                      uint32_t blockSize);
+/* fixTree3 from explicit start nodes (extension): the reference's viewer
+ * points GPUWorkingMemory.nodesToFix at the dynamic leaves and calls fixTree3
+ * (lib/viewer/LiveTracer.cpp:284-294); with the opaque working memory here
+ * the starts are passed directly.  Leaf starts are allowed. */
+bool bvht_fixTreeFrom(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, const uint32_t *devStarts,
+                      uint32_t num);
 /* CUDAcalcSAH -- lib/builder/Bittner13CUDA.cu:776-803: (sum SA - SA(root)) / SA(root) */
 float CUDAcalcSAH(bvht_CUDAMemoryBVHPointer *bvh);
 /* doCudaDevSync -- lib/builder/Bittner13CUDA.cu:824 */
