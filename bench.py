@@ -70,6 +70,17 @@ def profile_build(args):
     print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes}), flush=True)
 
 
+def measured_traffic(bits):
+    """HBM bytes per onesweep launch from the committed counter runs
+    (profiles/lbvh63_4M_traffic.md: FETCH_SIZE and WRITE_SIZE, calibrated
+    with a 256 MiB device copy); they cover the 63-bit workload only."""
+    path = os.path.join(ROOT, "profiles", "lbvh63_4M_traffic.json")
+    if bits != 63 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)["k_onesweep_mean_pass"]["traffic_bytes"]
+
+
 def run_bench(args):
     import bvhtest_amd as bt
 
@@ -128,7 +139,7 @@ def run_bench(args):
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": "k_onesweep (mean of %d passes)" % passes, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": int(dom_bytes), "ms_per_launch": round(dom_ms, 4)}
+                "traffic": measured_traffic(bits), "bytes_per_launch": int(dom_bytes), "ms_per_launch": round(dom_ms, 4)}
     print(f"# stages (ms): {json.dumps({k: round(x, 4) for k, x in stages.items()})}", file=sys.stderr, flush=True)
 
     # primary + bounce rays
