@@ -645,9 +645,12 @@ __global__ __launch_bounds__(256) void k_refitCount(const uint32_t *__restrict__
 }
 
 // kFixTree3_2 -- Bittner13CUDA.cu:425-467: the last arrival recomputes.
+// With leafCnt/subSA (the learned priority's subtree features) the same walk
+// keeps them exact: the nodes it recomputes are exactly those whose subtree
+// changed.
 __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__ starts, uint32_t num,
                                                     bvht_BVHNode *__restrict__ nodes, uint32_t N,
-                                                    uint32_t *__restrict__ cnt) {
+                                                    uint32_t *__restrict__ cnt, uint32_t *leafCnt, float *subSA) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= num) return;
   uint32_t node = starts[i];
@@ -668,7 +671,12 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
     bvht_AABB b = nodes[l].bbox;
     mergeInto(b, nodes[r].bbox);
     nodes[node].bbox = b;
-    nodes[node].surfaceArea = surfaceArea(b);
+    const float sa = surfaceArea(b);
+    nodes[node].surfaceArea = sa;
+    if (leafCnt) {
+      leafCnt[node] = leafCnt[l] + leafCnt[r];
+      subSA[node] = subSA[l] + subSA[r] + sa;
+    }
     fenceReleaseAgent();
     const uint32_t p = nodes[node].parent;
     if (p == node) return;
@@ -816,6 +824,8 @@ struct Internal {
   float *feat = nullptr;  // [numInner][12], inner-list order
   uint32_t *leafCnt = nullptr;
   float *subSA = nullptr;
+  bool subValid = false;  // leafCnt/subSA match the tree (kept by the refit)
+  uint32_t *anc[2] = {nullptr, nullptr}, *dep[2] = {nullptr, nullptr};
   PriorityModel *model = nullptr;
   float *recFeat = nullptr, *recGain = nullptr;
   uint32_t *recCount = nullptr;
@@ -825,12 +835,25 @@ struct Internal {
 bool features(Internal *in, const bvht_CUDAMemoryBVHPointer *bvh) {
   if (!in->feat) {
     if (!devAlloc(&in->feat, (size_t)in->numInner * kFeat, "bn13 features") ||
-        !devAlloc(&in->leafCnt, in->N, "bn13 leaf counts") || !devAlloc(&in->subSA, in->N, "bn13 subtree SA"))
+        !devAlloc(&in->leafCnt, in->N, "bn13 leaf counts") || !devAlloc(&in->subSA, in->N, "bn13 subtree SA") ||
+        !devAlloc(&in->anc[0], in->N, "bn13 ancestors") || !devAlloc(&in->anc[1], in->N, "bn13 ancestors") ||
+        !devAlloc(&in->dep[0], in->N, "bn13 depths") || !devAlloc(&in->dep[1], in->N, "bn13 depths"))
       return false;
   }
-  return computeFeatures(bvh->nodes, in->N, in->root, in->inner, in->numInner, in->leaves, in->numLeaves, in->leafCnt,
-                         in->subSA, in->refitCnt, in->feat);
+  if (!in->subValid) {
+    if (!computeSubtree(bvh->nodes, in->N, in->leaves, in->numLeaves, in->leafCnt, in->subSA, in->refitCnt))
+      return false;
+    in->subValid = true;
+  }
+  const int d = computeDepth(bvh->nodes, in->N, in->root, in->anc, in->dep);
+  if (d < 0) return false;
+  return computeFeatures(bvh->nodes, in->N, in->root, in->inner, in->numInner, in->leafCnt, in->subSA, in->dep[d],
+                         in->feat);
 }
+
+// Refit arguments that keep the subtree features (when they are in use).
+uint32_t *refitLeafCnt(const Internal *in) { return in->subValid ? in->leafCnt : nullptr; }
+float *refitSubSA(const Internal *in) { return in->subValid ? in->subSA : nullptr; }
 
 RecordSink sinkOf(const Internal *in) {
   RecordSink r{nullptr, in->recFeat, in->recGain, in->recCount, in->recMax};
@@ -960,6 +983,10 @@ void freeMemory(bvht_GPUWorkingMemory *data) {
     devFree(in->feat);
     devFree(in->leafCnt);
     devFree(in->subSA);
+    for (int b = 0; b < 2; ++b) {
+      devFree(in->anc[b]);
+      devFree(in->dep[b]);
+    }
     devFree(in->recFeat);
     devFree(in->recGain);
     devFree(in->recCount);
@@ -983,6 +1010,7 @@ void initData(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint3
       !check(hipStreamSynchronize(stream()), "initData sync"))
     return;
   in->root = hdr.vRootIndex;
+  in->subValid = false;
   const uint32_t selBlocks = divUp(in->N, kSelTile);
   k_selCount<<<selBlocks, kSelBlock, 0, stream()>>>(bvh->nodes, in->N, in->selCounts);
   if (!checkLaunch("k_selCount")) return;
@@ -1022,6 +1050,7 @@ void fixTree1(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint3
   (void)blockSize;
   Internal *in = inOf(data);
   if (!in || !validBVH(bvh, "fixTree1")) return;
+  in->subValid = false;  // boxes may change; the subtree features are recomputed when next needed
   refitFromLeaves(bvh->nodes, 0, in->numLeaves, in->leaves, in->refitCnt, in->N);
 }
 
@@ -1032,7 +1061,8 @@ void fixTree3(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint3
   const uint32_t num = 3 * in->K;
   k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
   if (!checkLaunch("k_refitCount")) return;
-  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
+  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt,
+                                                    refitLeafCnt(in), refitSubSA(in));
   checkLaunch("k_refitApply");
 }
 
@@ -1083,7 +1113,8 @@ bool bvht_fixTreeFrom(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bv
   }
   k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(devStarts, num, bvh->nodes, in->N, in->cnt);
   if (!checkLaunch("k_refitCount")) return false;
-  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(devStarts, num, bvh->nodes, in->N, in->cnt);
+  in->subValid = false;  // leaf boxes may have changed (dynamic refit)
+  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(devStarts, num, bvh->nodes, in->N, in->cnt, nullptr, nullptr);
   return checkLaunch("k_refitApply");
 }
 
@@ -1152,7 +1183,8 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
   const uint32_t num = 3 * K;
   k_refitCount<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
   if (!checkLaunch("k_refitCount")) return;
-  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt);
+  k_refitApply<<<divUp(num, 256), 256, 0, stream()>>>(in->fixStarts, num, bvh->nodes, in->N, in->cnt,
+                                                    refitLeafCnt(in), refitSubSA(in));
   checkLaunch("k_refitApply");
 }
 

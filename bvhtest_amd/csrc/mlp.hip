@@ -2,7 +2,10 @@
 // the fixed heuristic of Bittner13CUDA.cu:480-500).
 //
 //   k_subtreeUp  leaf count and subtree surface-area sum of every node
-//                (bottom-up, arrival counters between agent-scope fences);
+//                (bottom-up, arrival counters between agent-scope fences; the
+//                optimizer runs it once and then keeps both exact in its
+//                refit);
+//   k_depthInit / k_depthJump  node depths by pointer jumping;
 //   k_features   12 features per inner node (below);
 //   k_mlpInfer   12-64-64-1 MLP, ReLU, BF16 operands on
 //                v_mfma_f32_16x16x16_bf16 with FP32 accumulation, 16 nodes
@@ -21,7 +24,7 @@
 //   0 log2(S/Sr)/16          1 L/S                 2 R/S
 //   3 S/P (1 at the root)    4 SA(sibling)/P       5 min(L,R)/(L+R)
 //   6 log2(c/Sr)/16 with c = 2 S^3 / ((L+R) min(L,R)), the reference priority
-//   7 depth/64               8 log2(leaves below x)/24
+//   7 min(depth, 255)/64     8 log2(leaves below x)/24
 //   9 vol(B_l n B_r)/vol(B_x)   10 vol(B_x n B_sibling)/vol(B_parent)
 //   11 log2(subtree SA sum / (S * leaves))/8
 #include "mlp.hpp"
@@ -89,10 +92,38 @@ __global__ __launch_bounds__(256) void k_subtreeUp(const bvht_BVHNode *__restric
   }
 }
 
+// Pointer jumping: after round r, anc is the ancestor 2^r steps up (or the
+// root) and dep the distance to it.
+__global__ __launch_bounds__(256) void k_depthInit(const bvht_BVHNode *__restrict__ nodes, uint32_t N, uint32_t root,
+                                                   uint32_t *__restrict__ anc, uint32_t *__restrict__ dep) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t p = nodes[i].parent;
+  const bool top = i == root || p >= N;
+  anc[i] = top ? root : p;
+  dep[i] = top ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(256) void k_depthJump(uint32_t N, uint32_t root, const uint32_t *__restrict__ ancIn,
+                                                   const uint32_t *__restrict__ depIn, uint32_t *__restrict__ ancOut,
+                                                   uint32_t *__restrict__ depOut) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t a = ancIn[i];
+  if (a == root) {
+    ancOut[i] = a;
+    depOut[i] = depIn[i];
+  } else {
+    ancOut[i] = ancIn[a];
+    depOut[i] = depIn[i] + depIn[a];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_features(const bvht_BVHNode *__restrict__ nodes, uint32_t N, uint32_t root,
                                                   const uint32_t *__restrict__ inner, uint32_t num,
                                                   const uint32_t *__restrict__ leafCnt,
-                                                  const float *__restrict__ subSA, float *__restrict__ feat) {
+                                                  const float *__restrict__ subSA, const uint32_t *__restrict__ depth,
+                                                  float *__restrict__ feat) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= num) return;
   const uint32_t x = inner[i];
@@ -118,12 +149,7 @@ __global__ __launch_bounds__(256) void k_features(const bvht_BVHNode *__restrict
     f[4] = safeDiv(sn.surfaceArea, pn.surfaceArea);
     f[10] = safeDiv(overlapVol(nd.bbox, sn.bbox), boxVol(pn.bbox));
   }
-  uint32_t depth = 0;
-  for (uint32_t y = x; y != root && depth < 255; ++depth) {
-    y = nodes[y].parent;
-    if (y >= N) break;
-  }
-  f[7] = (float)depth * (1.0f / 64.0f);
+  f[7] = (float)min(depth[x], 255u) * (1.0f / 64.0f);
   const float lc = (float)leafCnt[x];
   f[8] = lg2(lc) * (1.0f / 24.0f);
   f[9] = safeDiv(overlapVol(nodes[nd.left].bbox, nodes[nd.right].bbox), boxVol(nd.bbox));
@@ -449,13 +475,29 @@ void freeModel(PriorityModel *m) {
 
 }  // namespace
 
-bool computeFeatures(const bvht_BVHNode *nodes, uint32_t N, uint32_t root, const uint32_t *inner, uint32_t numInner,
-                     const uint32_t *leaves, uint32_t numLeaves, uint32_t *leafCnt, float *subSA, uint32_t *arrive,
-                     float *feat) {
-  if (numInner == 0) return true;
+bool computeSubtree(const bvht_BVHNode *nodes, uint32_t N, const uint32_t *leaves, uint32_t numLeaves,
+                    uint32_t *leafCnt, float *subSA, uint32_t *arrive) {
+  if (numLeaves == 0) return true;
   k_subtreeUp<<<divUp(numLeaves, 256), 256, 0, stream()>>>(nodes, N, leaves, numLeaves, leafCnt, subSA, arrive);
-  if (!checkLaunch("k_subtreeUp") || !fillU32(arrive, 0u, N)) return false;
-  k_features<<<divUp(numInner, 256), 256, 0, stream()>>>(nodes, N, root, inner, numInner, leafCnt, subSA, feat);
+  return checkLaunch("k_subtreeUp") && fillU32(arrive, 0u, N);
+}
+
+int computeDepth(const bvht_BVHNode *nodes, uint32_t N, uint32_t root, uint32_t *anc[2], uint32_t *dep[2]) {
+  k_depthInit<<<divUp(N, 256), 256, 0, stream()>>>(nodes, N, root, anc[0], dep[0]);
+  if (!checkLaunch("k_depthInit")) return -1;
+  int cur = 0;
+  for (int r = 0; r < kDepthRounds; ++r) {
+    k_depthJump<<<divUp(N, 256), 256, 0, stream()>>>(N, root, anc[cur], dep[cur], anc[cur ^ 1], dep[cur ^ 1]);
+    if (!checkLaunch("k_depthJump")) return -1;
+    cur ^= 1;
+  }
+  return cur;
+}
+
+bool computeFeatures(const bvht_BVHNode *nodes, uint32_t N, uint32_t root, const uint32_t *inner, uint32_t numInner,
+                     const uint32_t *leafCnt, const float *subSA, const uint32_t *depth, float *feat) {
+  if (numInner == 0) return true;
+  k_features<<<divUp(numInner, 256), 256, 0, stream()>>>(nodes, N, root, inner, numInner, leafCnt, subSA, depth, feat);
   return checkLaunch("k_features");
 }
 

@@ -38,11 +38,19 @@ struct PriorityModel {
   uint32_t step = 0;
 };
 
-// Features of the inner nodes `inner[0..num)` into feat[num][12].  Scratch:
-// leafCnt/subSA/arrive of N entries (arrive zero on entry and on exit).
+// Leaf count and subtree surface-area sum of every node (bottom-up from the
+// leaves; `arrive` holds N zeros on entry and on exit).
+bool computeSubtree(const bvht_BVHNode *nodes, uint32_t N, const uint32_t *leaves, uint32_t numLeaves,
+                    uint32_t *leafCnt, float *subSA, uint32_t *arrive);
+// Depth of every node, exact below 2^kDepthRounds (larger depths come out
+// as at least that), by pointer jumping over double buffers anc[2][N],
+// dep[2][N]; returns the buffer index holding the result (-1 on error).
+constexpr int kDepthRounds = 8;
+int computeDepth(const bvht_BVHNode *nodes, uint32_t N, uint32_t root, uint32_t *anc[2], uint32_t *dep[2]);
+// Features of the inner nodes `inner[0..num)` into feat[num][12] from the
+// per-node leaf counts, subtree SA sums and depths.
 bool computeFeatures(const bvht_BVHNode *nodes, uint32_t N, uint32_t root, const uint32_t *inner, uint32_t numInner,
-                     const uint32_t *leaves, uint32_t numLeaves, uint32_t *leafCnt, float *subSA, uint32_t *arrive,
-                     float *feat);
+                     const uint32_t *leafCnt, const float *subSA, const uint32_t *depth, float *feat);
 
 // Model outputs of feat[num][12] as descending sort keys (~order-preserving
 // bits) plus the per-pass digit histograms of a 32-bit sort, or as raw

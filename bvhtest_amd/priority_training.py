@@ -5,8 +5,9 @@ Records come from the plain GPU optimizer itself (1% of the nodes per step,
 clusters + triangle soup).  Every patch a step evaluates is recorded as the
 features of its node at selection time and the SAH gain the searches found for
 it.  Each step also probes uniformly sampled inner nodes, so the model sees
-nodes the heuristic would not have picked.  Targets are z-scored, and ranking is
-invariant to that.
+nodes the heuristic would not have picked.  The regression target is an
+increasing map of the gain (z-score, asinh or rank; see transform_target):
+the optimizer only uses the order of the model's outputs.
 
 The model is trained with Adam (hand-written BF16-MFMA kernels, bvht_mlpTrain).
 Evaluation is time to reach the plain optimizer's final SAH(1.2, 1) within
@@ -23,7 +24,7 @@ import time
 import numpy as np
 
 from . import (Bittner13GPU, BVHTestError, DeviceMesh, PriorityModel, calc_sah, lbvh_build, lib,
-               scene_spheres_soup)
+               scene_spheres_soup, validate_device)
 
 
 def scene(seed, spheres, soup):
@@ -59,9 +60,27 @@ def generate_records(seeds, spheres, soup, steps, probes_per_step, batch_percent
     return x[keep], g[keep]
 
 
-def train(x, g, seed=0, epochs=(12, 6, 4), lrs=(2e-3, 5e-4, 1e-4), batch=4096, log=None):
+def transform_target(g, kind):
+    """Regression target from the SAH gains.  Only the order of the model's
+    outputs matters to the optimizer, so any increasing map of the gain is a
+    valid target: "z" z-scores the gains, "asinh" compresses their heavy tail
+    first (scale: the median absolute gain), "rank" replaces them by their
+    ranks spread over [-1, 1]."""
+    g = np.asarray(g, np.float64)
+    if kind == "asinh":
+        scale = float(np.median(np.abs(g))) or 1.0
+        g = np.arcsinh(g / scale)
+    elif kind == "rank":
+        r = np.argsort(np.argsort(g, kind="stable"), kind="stable")
+        return (2.0 * r / max(len(g) - 1, 1) - 1.0).astype(np.float32), {"target": kind}
+    elif kind != "z":
+        raise ValueError(f"unknown target transform {kind!r}")
     mu, sd = float(g.mean()), float(g.std() + 1e-12)
-    t = ((g - mu) / sd).astype(np.float32)
+    return ((g - mu) / sd).astype(np.float32), {"target": kind, "target_mean": mu, "target_std": sd}
+
+
+def train(x, g, seed=0, epochs=(12, 6, 4), lrs=(2e-3, 5e-4, 1e-4), batch=4096, target="asinh", log=None):
+    t, info = transform_target(g, target)
     model = PriorityModel(seed=seed)
     for e, lr in zip(epochs, lrs):
         loss = model.train(x, t, epochs=e, batch=batch, lr=lr, seed=seed + e)
@@ -69,7 +88,8 @@ def train(x, g, seed=0, epochs=(12, 6, 4), lrs=(2e-3, 5e-4, 1e-4), batch=4096, l
             log(f"train: {e} epochs at lr {lr}: loss {loss:.4f}")
     pred = model.predict(x)
     rank = np.corrcoef(np.argsort(np.argsort(pred)), np.argsort(np.argsort(t)))[0, 1]
-    return model, {"target_mean": mu, "target_std": sd, "train_rank_corr": float(rank), "records": int(len(t))}
+    info.update(train_rank_corr=float(rank), records=int(len(t)))
+    return model, info
 
 
 def _timed_step(opt, tree):
@@ -118,8 +138,10 @@ def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=
         prev = sah
         if sah <= target:
             break
+    total, _ = validate_device(tree, mesh)
     res.update(learned_steps=steps, learned_s=tl, learned_sah=sah, fallback_steps=fallbacks,
-               reached=bool(sah <= target), time_ratio=(tl / tp) if sah <= target else None)
+               reached=bool(sah <= target), time_ratio=(tl / tp) if sah <= target else None,
+               learned_tree_valid=total == 0)
     opt.free()
     tree.free()
     return res
@@ -137,6 +159,9 @@ def main(argv=None):
     ap.add_argument("--eval-spheres", type=int, default=2048)
     ap.add_argument("--eval-soup", type=int, default=500000)
     ap.add_argument("--eval-steps", type=int, default=50)
+    ap.add_argument("--target", default="asinh", choices=("z", "asinh", "rank"))
+    ap.add_argument("--save-records", help="write a random subsample of the records (.npz: x, g)")
+    ap.add_argument("--save-count", type=int, default=200000)
     args = ap.parse_args(argv)
 
     def seeds(spec):
@@ -147,7 +172,10 @@ def main(argv=None):
         print(msg, file=sys.stderr, flush=True)
 
     x, g = generate_records(seeds(args.train_seeds), args.spheres, args.soup, args.steps, args.probes, log=log)
-    model, info = train(x, g, log=log)
+    if args.save_records:
+        keep = np.random.default_rng(0).permutation(len(g))[: args.save_count]
+        np.savez_compressed(args.save_records, x=x[keep], g=g[keep])
+    model, info = train(x, g, target=args.target, log=log)
     model.save(args.out)
     log(f"saved {args.out}: {json.dumps(info)}")
     for s in seeds(args.eval_seeds):

@@ -236,6 +236,11 @@ def test_learned_selection_keeps_a_valid_tree(bvh, orc):
     total, err = orc.validate(nodes, 0, ml, v, f)
     assert total == 0, err
     assert bvh.calc_sah(tree) < s0
+    # the subtree leaf counts and SA sums were carried through the steps'
+    # refits, not recomputed: the features still match a fresh evaluation
+    got = opt.features(tree)
+    ref = features_ref(nodes)
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-6), np.abs(got - ref).max(axis=0)
     # selection without a model is refused
     bare = bvh.Bittner13GPU(maxNumStepps=1, numChunks=8, batchPercent=3.0, priority=1)
     bare.setup(tree, len(f))
