@@ -32,18 +32,33 @@ HBM_PEAK_GBS = 8000.0
 CAM = dict(pos=(0.5, 0.5, 2.0), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov=45.0)
 
 
-def kernel_bytes(name, n, morton_bits, pass_idx, passes):
-    """Algorithmic HBM bytes of one launch, per the byte model of the build:
-    scene box: face 12 + 3 vertex gathers counted once per vertex (12 V/n);
-    Morton: face 12 + vertices 12 V/n + key write; onesweep pass: key + value
-    read and written (first pass reads no value, last pass writes no key);
-    level-0 build: sorted index 4 + face 12 + vertices, leaf 48 + inner 48
-    written per triangle."""
+SORT_STAGES = ("k_msdScan", "k_msdScatter", "k_msdWindows", "k_msdBig", "k_onesweep")
+
+
+def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
+    """Algorithmic HBM bytes of one launch, per the byte model of the build
+    (n triangles, nv vertices, each vertex counted once):
+    scene box: faces 12 n + vertices 12 nv; Morton: the same plus the key
+    write; MSD scatter: key read, key + index written; onesweep pass: key +
+    value read and written (first pass reads no value, last pass writes no
+    key); level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
+    48-byte leaves (n) and the inner nodes below the top tiles
+    (n - ceil(n / 2^tile_log2)) written.  The MSD window and big-bucket sorts
+    move a data-dependent share of the pairs, so they have no byte model."""
     kb = 8 if morton_bits == 63 else 4
-    if name == "k_onesweep":
+    if name == "k_sceneBox":
+        return 12 * n + 12 * nv
+    if name in ("k_morton", "k_mortonMsd"):
+        return 12 * n + 12 * nv + kb * n
+    if name == "k_msdScatter":
+        return kb * n + (kb + 4) * n
+    if name.startswith("k_onesweep"):
         rd = kb + (0 if pass_idx == 0 else 4)
         wr = (0 if pass_idx == passes - 1 else kb) + 4
         return n * (rd + wr)
+    if name == "k_buildLevel0":
+        inner = n - -(-n // (1 << tile_log2))
+        return 16 * n + 12 * nv + 48 * (n + inner)
     return None
 
 
@@ -56,7 +71,7 @@ def profile_build(args):
     L = bt.lib()
     v, f = bt.scene_spheres_soup(2048, 500000, 2)
     mesh = bt.DeviceMesh(v, f)
-    builder = bt.LBVHBuilder(mesh, args.morton_bits)
+    builder = bt.LBVHBuilder(mesh, args.morton_bits, digit_bits=args.digit_bits)
     tree = builder.allocate_bvh()
     for _ in range(args.steps):
         builder.build(tree)
@@ -70,15 +85,16 @@ def profile_build(args):
     print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes}), flush=True)
 
 
-def measured_traffic(bits):
-    """HBM bytes per onesweep launch from the committed counter runs
-    (profiles/lbvh63_4M_traffic.md: FETCH_SIZE and WRITE_SIZE, calibrated
+def measured_traffic(kernel, bits):
+    """HBM bytes per launch of `kernel` from the committed counter runs
+    (profiles/lbvh63_4M_traffic.json: FETCH_SIZE and WRITE_SIZE, calibrated
     with a 256 MiB device copy); they cover the 63-bit workload only."""
     path = os.path.join(ROOT, "profiles", "lbvh63_4M_traffic.json")
     if bits != 63 or not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh)["k_onesweep_mean_pass"]["traffic_bytes"]
+        entry = json.load(fh).get("kernels", {}).get(kernel)
+    return entry["traffic_bytes"] if entry else None
 
 
 def run_bench(args):
@@ -95,7 +111,7 @@ def run_bench(args):
     print(f"# scene: {n} triangles, {len(v)} vertices ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     mesh = bt.DeviceMesh(v, f)
     bits = args.morton_bits
-    builder = bt.LBVHBuilder(mesh, bits)
+    builder = bt.LBVHBuilder(mesh, bits, digit_bits=args.digit_bits)
     tree = builder.allocate_bvh()
 
     # warmup
@@ -131,15 +147,25 @@ def run_bench(args):
     L.bvht_LBVH_setTiming(False)
     passes = sum(1 for k in per if k.startswith("k_onesweep"))
     stages = {k: float(np.mean(x)) for k, x in per.items()}
-    # dominant kernel family: the onesweep passes
-    sort_ms = sum(vv for kk, vv in stages.items() if kk.startswith("k_onesweep"))
-    sort_bytes = sum(kernel_bytes("k_onesweep", n, bits, p, passes) for p in range(passes))
-    dom_ms = sort_ms / passes
-    dom_bytes = sort_bytes / passes
+    stages["sort_total"] = sum(x for k, x in stages.items() if k.startswith(SORT_STAGES))
+    nv = len(v)
+    tile_log2 = builder.mem.opts.tileLog2
+
+    def nbytes(name):
+        if name.startswith("k_onesweep"):
+            p = int(name[len("k_onesweep["):-1])
+            return kernel_bytes("k_onesweep", n, nv, bits, tile_log2, p, passes)
+        return kernel_bytes(name, n, nv, bits, tile_log2)
+
+    # dominant kernel: the longest stage that has a byte model
+    modelled = [(ms, name) for name, ms in stages.items() if name != "sort_total" and nbytes(name)]
+    dom_ms, dom = max(modelled)
+    dom_bytes = nbytes(dom)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_onesweep (mean of %d passes)" % passes, "achieved": round(achieved, 1),
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(bits), "bytes_per_launch": int(dom_bytes), "ms_per_launch": round(dom_ms, 4)}
+                "traffic": measured_traffic(dom, bits), "bytes_per_launch": int(dom_bytes),
+                "ms_per_launch": round(dom_ms, 4)}
     print(f"# stages (ms): {json.dumps({k: round(x, 4) for k, x in stages.items()})}", file=sys.stderr, flush=True)
 
     # primary + bounce rays
@@ -211,7 +237,9 @@ def run_bench(args):
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "icosphere clusters + triangle soup, LBVH build + refit, 1920x1080 primary + "
                                   "1 diffuse bounce", "triangles": n, "spheres": 2048, "soup": 500000,
-                      "morton_bits": bits, "topology": "reference (index Karras)", "seed": 2},
+                      "morton_bits": bits, "topology": "reference (index Karras)", "seed": 2,
+                      "sort": "onesweep, 8-bit digits" if builder.mem.opts.digitBits == 8 else
+                              "MSD: 16-bit buckets, in-LDS sorts by (key, index)"},
            "roofline": roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
            "trace": trace, "bittner": bittner, "batched": batch, "learned_priority": learned, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
@@ -264,16 +292,17 @@ def batch_bench(bt, L, reps=5):
     return out
 
 
-def learned_bench(bt, seed=200, plain_steps=50):
-    """Time to reach the plain optimizer's final SAH(1.2, 1) within 0.5%:
-    learned priority vs. plain, on a held-out 4.2M-triangle scene."""
+def learned_bench(bt, seed=200, plain_steps=100):
+    """Time to reach the plain optimizer's final SAH(1.2, 1) after 100 steps
+    within 0.5%: learned priority vs. plain, on one held-out ~16.9M-triangle
+    scene (8,192 icospheres + 2M soup triangles)."""
     path = os.path.join(ROOT, "bvhtest_amd", "data", "priority_mlp.npy")
     if not os.path.exists(path):
         return {"status": "no trained weights at bvhtest_amd/data/priority_mlp.npy"}
     from bvhtest_amd import priority_training as pt
 
     model = bt.PriorityModel.load(path)
-    r = pt.evaluate(model, seed, 2048, 500000, plain_steps)
+    r = pt.evaluate(model, seed, 8192, 2000000, plain_steps)
     r["weights"] = "bvhtest_amd/data/priority_mlp.npy"
     print(f"# learned priority: {r}", file=sys.stderr, flush=True)
     return r
@@ -314,6 +343,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--morton-bits", type=int, default=63, choices=[30, 63])
+    ap.add_argument("--digit-bits", type=int, default=0, choices=[0, 8, 16],
+                    help="LBVH sort: 8 (default, 0) onesweep, 16 MSD")
     ap.add_argument("--bittner-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

@@ -167,11 +167,12 @@ def download_u64(dev_ptr, count):
 class LBVHBuilder:
     """Working memory of the LBVH command (lib/builder/LBVH.cpp:28-55)."""
 
-    def __init__(self, mesh, morton_bits=30, topology=0):
+    def __init__(self, mesh, morton_bits=30, topology=0, digit_bits=0):
         self.mesh = mesh
         opts = lib().bvht_LBVH_defaultOptions()
         opts.mortonBits = morton_bits
         opts.topology = topology
+        opts.digitBits = digit_bits
         self.mem = lib().bvht_LBVH_allocateWorkingMemoryEx(ctypes.byref(mesh.raw), ctypes.byref(opts))
         if not self.mem.lRes:
             _fail("LBVH_allocateWorkingMemory")
@@ -208,6 +209,10 @@ class LBVHBuilder:
         self._check("LBVH_doCUDASyc")
         return np.array([[box.minMax[0].x, box.minMax[0].y, box.minMax[0].z],
                          [box.minMax[1].x, box.minMax[1].y, box.minMax[1].z]], np.float32)
+
+    def sort_status(self):
+        """Error bits of the last sort (0: completed)."""
+        return lib().bvht_LBVH_sortStatus(ctypes.byref(self.mem))
 
     def sorted_faces(self):
         lib().LBVH_doCUDASyc()
@@ -253,6 +258,20 @@ def vec3(x):
 # ---------------------------------------------------------------------------
 # tracer (CUDATracer command, lib/tracer/CUDATracer.cpp:75-150)
 # ---------------------------------------------------------------------------
+
+def sort_key_index_pairs(keys, key_bits, digit_bits=0):
+    """(indices, sorted keys) of the LBVH build's sort on host keys (uint32
+    for 30 bits, uint64 for 63)."""
+    keys = np.ascontiguousarray(keys, np.uint64 if key_bits == 63 else np.uint32)
+    n = len(keys)
+    kin = DeviceBuffer(keys.nbytes).upload(keys)
+    kout = DeviceBuffer(keys.nbytes)
+    vout = DeviceBuffer(4 * n)
+    if not lib().bvht_sortKeyIndexPairs(kin.ptr, n, key_bits, digit_bits, kout.ptr,
+                                        ctypes.cast(vout.ptr, ctypes.POINTER(ctypes.c_uint32))):
+        _fail("bvht_sortKeyIndexPairs")
+    return vout.download(np.uint32, n), kout.download(keys.dtype, n)
+
 
 class DeviceBuffer:
     """Plain device allocation (runMalloc / runFree)."""

@@ -110,6 +110,8 @@ SIGNATURES = {
     "LBVH_freeWorkingMemory": (None, [P(LBVHWorkingMemory)]),
     "LBVH_doCUDASyc": (None, []),
     "bvht_LBVH_build": (c_bool, [P(LBVHWorkingMemory), P(MeshRaw), P(CUDAMemoryBVHPointer)]),
+    "bvht_LBVH_sortStatus": (c_u32, [P(LBVHWorkingMemory)]),
+    "bvht_sortKeyIndexPairs": (c_bool, [c_vp, c_u32, c_u32, c_u32, c_vp, P(c_u32)]),
     "bvht_LBVH_setTiming": (None, [c_bool]),
     "bvht_LBVH_stageTimes": (ctypes.c_int, [P(c_f), ctypes.c_int]),
     "bvht_LBVH_stageName": (ctypes.c_char_p, [ctypes.c_int]),

@@ -17,6 +17,7 @@
 // read to build it and no global atomic is needed for the refit.
 #include "lbvh.hpp"
 #include "lbvh_kernels.hpp"
+#include "msd_sort.hpp"
 #include "radix_sort.hpp"
 #include "refit.hpp"
 #include "wave.hpp"
@@ -42,15 +43,34 @@ __global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict
   for (size_t i = (size_t)blockIdx.x * kBlock + tid; i < zeroWords; i += (size_t)gridDim.x * kBlock) zeroPtr[i] = 0;
   // the reduction starts from the all-zero box (LBVH_CUDA.cu:104-106)
   float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
-  for (uint32_t i = blockIdx.x * kBlock + tid; i < n; i += gridDim.x * kBlock) {
-    bvht_Triangle t = faces[i];
-    bvht_AABB b = triangleBox(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
-    mn[0] = mergeMin(mn[0], b.minMax[0].x);
-    mn[1] = mergeMin(mn[1], b.minMax[0].y);
-    mn[2] = mergeMin(mn[2], b.minMax[0].z);
-    mx[0] = mergeMax(mx[0], b.minMax[1].x);
-    mx[1] = mergeMax(mx[1], b.minMax[1].y);
-    mx[2] = mergeMax(mx[2], b.minMax[1].z);
+  // four triangles per trip, every load issued before the first is used
+  constexpr int R = 4;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t i0 = blockIdx.x * kBlock + tid; i0 < n; i0 += R * stride) {
+    bvht_Triangle t[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const uint32_t i = i0 + u * stride;
+      t[u] = i < n ? faces[i] : faces[i0];
+    }
+    bvht_vec3 a[R], b[R], c[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      a[u] = ldVec3(vert + t[u].v1);
+      b[u] = ldVec3(vert + t[u].v2);
+      c[u] = ldVec3(vert + t[u].v3);
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      // a repeat of triangle i0 for a slot past the end changes no min/max
+      const bvht_AABB bx = triangleBox(a[u], b[u], c[u]);
+      mn[0] = mergeMin(mn[0], bx.minMax[0].x);
+      mn[1] = mergeMin(mn[1], bx.minMax[0].y);
+      mn[2] = mergeMin(mn[2], bx.minMax[0].z);
+      mx[0] = mergeMax(mx[0], bx.minMax[1].x);
+      mx[1] = mergeMax(mx[1], bx.minMax[1].y);
+      mx[2] = mergeMax(mx[2], bx.minMax[1].z);
+    }
   }
   __shared__ float s_red[6][kBlock / 64];
   __shared__ bool s_last;
@@ -144,6 +164,50 @@ __global__ __launch_bounds__(kBlock) void k_morton(const bvht_vec3 *__restrict__
   }
 }
 
+// Morton keys plus the top-digit histogram of the MSD sort.  A wave takes
+// kMortonRuns groups of 64 consecutive triangles and issues every face and
+// vertex load of them before computing, so each wave has 4x the loads in
+// flight of a one-triangle loop.
+constexpr int kMortonRuns = 4;
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_mortonMsd(const bvht_vec3 *__restrict__ vert,
+                                                      const bvht_Triangle *__restrict__ faces, uint32_t n,
+                                                      const BuildParams *__restrict__ prm, K *__restrict__ keys,
+                                                      uint32_t *__restrict__ hist, int shift) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bvht_vec3 off = {prm->offset[0], prm->offset[1], prm->offset[2]};
+  const bvht_vec3 sc = {prm->scale[0], prm->scale[1], prm->scale[2]};
+  const uint32_t stride = gridDim.x * kBlock * kMortonRuns;
+  for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * 64 * kMortonRuns; base < n; base += stride) {
+    bvht_Triangle t[kMortonRuns];
+#pragma unroll
+    for (int u = 0; u < kMortonRuns; ++u) {
+      const uint32_t i = base + u * 64 + lane;
+      t[u] = i < n ? faces[i] : bvht_Triangle{0, 0, 0};
+    }
+    bvht_vec3 a[kMortonRuns], b[kMortonRuns], c[kMortonRuns];
+#pragma unroll
+    for (int u = 0; u < kMortonRuns; ++u) {
+      a[u] = ldVec3(vert + t[u].v1);
+      b[u] = ldVec3(vert + t[u].v2);
+      c[u] = ldVec3(vert + t[u].v3);
+    }
+#pragma unroll
+    for (int u = 0; u < kMortonRuns; ++u) {
+      const uint32_t i = base + u * 64 + lane;
+      const bool valid = i < n;
+      const bvht_vec3 ce = centroid(a[u], b[u], c[u]);
+      K code;
+      if constexpr (sizeof(K) == 8)
+        code = morton63(ce, off, sc);
+      else
+        code = morton30(ce, off, sc);
+      if (valid) keys[i] = code;
+      if (base + u * 64 < n) msdCountRun(hist, (uint32_t)(code >> shift) & (kMsdBuckets - 1), valid);
+    }
+  }
+}
+
 // Karras 2012 over the whole key (morton, index) -- extension (topology 1).
 // delta(i, j) = common prefix length of (key_i, i) and (key_j, j).
 template <typename K>
@@ -214,7 +278,9 @@ __global__ __launch_bounds__(kBlock) void k_leaves(bvht_BVHNode *__restrict__ no
 }
 
 struct Internal {
-  SortPlan plan;
+  SortPlan plan;  // onesweep (digitBits 8, the default)
+  MsdPlan msd;    // MSD sort (digitBits 16)
+  bool useMsd = false;
   float *partials = nullptr;
   uint32_t boxBlocks = 0;
   int resultBuf = 0;
@@ -273,9 +339,25 @@ uint32_t mortonBlocks(uint32_t n) {
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
+// Words of the sort scratch that must be zero before the key producer runs.
+size_t sortZeroWords(const Internal *in) { return in->useMsd ? in->msd.zeroWords : in->plan.totalWords; }
+
 bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
   Internal *in = internalOf(mem);
   uint32_t n = mesh->numFaces;
+  if (in->useMsd) {
+    uint32_t *hist = msdHist(in->msd, (uint32_t *)mem->sortScratch);
+    uint32_t blocks = divUp(n, kBlock * kMortonRuns);
+    blocks = std::max<uint32_t>(1, std::min<uint32_t>(blocks, (uint32_t)numCUs() * 8));
+    const int shift = (int)mem->opts.mortonBits - kMsdTopBits;
+    if (mem->opts.mortonBits == 63)
+      k_mortonMsd<uint64_t><<<blocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
+                                                              (uint64_t *)mem->keys[0], hist, shift);
+    else
+      k_mortonMsd<uint32_t><<<blocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
+                                                              (uint32_t *)mem->keys[0], hist, shift);
+    return checkLaunch("k_mortonMsd");
+  }
   uint32_t *hist = sortHist(in->plan, (uint32_t *)mem->sortScratch);
   if (mem->opts.mortonBits == 63)
     k_morton<uint64_t, 8><<<mortonBlocks(n), kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
@@ -290,6 +372,16 @@ bool runSort(bvht_LBVH_WorkingMemory *mem) {
   Internal *in = internalOf(mem);
   const bool keepKeys = mem->opts.topology == 1;
   Timing &tm = timing();
+  if (in->useMsd) {
+    // pairs end in keys[0] / values[1]; keys[0] doubles as split scratch
+    if (!msdRun(in->msd, mem->keys[0], mem->keys[1], mem->values[0], mem->keys[0], mem->values[1], keepKeys,
+                (uint32_t *)mem->sortScratch, tm.enabled ? [](const char *s) { timing().record(s); } : nullptr))
+      return false;
+    in->resultBuf = 1;
+    mem->sortedValues = mem->values[1];
+    mem->sortedKeys = keepKeys ? mem->keys[0] : nullptr;
+    return true;
+  }
   int r = sortRun(in->plan, mem->keys, mem->values, true, keepKeys, (uint32_t *)mem->sortScratch,
                   tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr);
   if (r < 0) return false;
@@ -448,7 +540,12 @@ bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *me
     setError("LBVH_allocateWorkingMemory: tileLog2 must be in [8, 11] (got %u)", m.opts.tileLog2);
     return m;
   }
-  m.opts.digitBits = kSortRadixBits;
+  if (m.opts.digitBits == 0) m.opts.digitBits = kSortRadixBits;
+  if (m.opts.digitBits != kMsdTopBits && m.opts.digitBits != kSortRadixBits) {
+    setError("LBVH_allocateWorkingMemory: digitBits must be %d (MSD sort) or %d (onesweep) (got %u)", kMsdTopBits,
+             kSortRadixBits, m.opts.digitBits);
+    return m;
+  }
   if (!validateMesh(mesh, "LBVH_allocateWorkingMemory")) return m;
   const uint32_t n = mesh->numFaces;
   const int keyBytes = m.opts.mortonBits == 63 ? 8 : 4;
@@ -459,9 +556,15 @@ bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *me
   }
   m.internal = in;
   m.numFaces = n;
-  in->plan = makeSortPlan(n, keyBytes, (int)m.opts.mortonBits);
+  in->useMsd = m.opts.digitBits == kMsdTopBits;
+  if (in->useMsd) {
+    in->msd = makeMsdPlan(n, keyBytes, (int)m.opts.mortonBits);
+    m.sortScratchBytes = msdScratchBytes(in->msd);
+  } else {
+    in->plan = makeSortPlan(n, keyBytes, (int)m.opts.mortonBits);
+    m.sortScratchBytes = sortScratchBytes(in->plan);
+  }
   in->boxBlocks = boxBlocks(n);
-  m.sortScratchBytes = sortScratchBytes(in->plan);
   m.numLocks = 2 * n - 1;
   if (!devMalloc(&m.keys[0], (size_t)keyBytes * n, "LBVH keys") || !devMalloc(&m.keys[1], (size_t)keyBytes * n, "LBVH keys") ||
       !devAlloc(&m.values[0], n, "LBVH values") || !devAlloc(&m.values[1], n, "LBVH values") ||
@@ -524,7 +627,7 @@ bvht_AABB LBVH_initTriData(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mes
   in->mesh = *mesh;
   in->haveMesh = true;
   k_sceneBox<<<in->boxBlocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
-                                                     in->partials, (uint32_t *)mem->sortScratch, in->plan.totalWords);
+                                                     in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
   if (!checkLaunch("k_sceneBox")) return box;
   BuildParams p;
   if (check(hipMemcpyAsync(&p, mem->params, sizeof(p), hipMemcpyDeviceToHost, stream()), "LBVH_initTriData box") &&
@@ -557,7 +660,7 @@ void LBVH_calcMortonCodes(bvht_LBVH_WorkingMemory *mem, bvht_AABB sceneAABB) {
   if (!check(hipMemcpyAsync(mem->params, &in->hostParams, sizeof(p), hipMemcpyHostToDevice, stream()),
              "LBVH_calcMortonCodes params"))
     return;
-  if (!sortReset(in->plan, (uint32_t *)mem->sortScratch)) return;
+  if (!fillU32((uint32_t *)mem->sortScratch, 0u, sortZeroWords(in))) return;
   launchMorton(mem, &in->mesh);
 }
 
@@ -608,7 +711,7 @@ bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvh
   tm.start();
   tm.record("begin");
   k_sceneBox<<<in->boxBlocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
-                                                     in->partials, (uint32_t *)mem->sortScratch, in->plan.totalWords);
+                                                     in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
   if (!checkLaunch("k_sceneBox")) return false;
   tm.record("k_sceneBox");
   if (!launchMorton(mem, mesh)) return false;
@@ -616,6 +719,21 @@ bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvh
   if (!runSort(mem)) return false;
   if (!runTree(mem, mesh, bvh)) return false;
   return true;
+}
+
+uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem) {
+  if (!mem || !mem->lRes || !mem->internal) {
+    setError("bvht_LBVH_sortStatus: working memory not allocated");
+    return 0xFFFFFFFFu;
+  }
+  const Internal *in = internalOf(mem);
+  const size_t word = in->useMsd ? in->msd.ctrlOff + 2 : in->plan.counterOff + 8;
+  uint32_t v = 0xFFFFFFFFu;
+  if (!check(hipMemcpyAsync(&v, (uint32_t *)mem->sortScratch + word, 4, hipMemcpyDeviceToHost, stream()),
+             "bvht_LBVH_sortStatus") ||
+      !check(hipStreamSynchronize(stream()), "bvht_LBVH_sortStatus sync"))
+    return 0xFFFFFFFFu;
+  return v;
 }
 
 void bvht_LBVH_setTiming(bool enable) { timing().enabled = enable; }

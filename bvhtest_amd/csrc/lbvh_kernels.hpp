@@ -83,20 +83,50 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__rest
   if (u0 >= U) return;  // block-uniform: a group past the last unit owns nothing
   const uint32_t cnt = (uint32_t)min((uint64_t)T, U - u0);
 
-  for (uint32_t k = threadIdx.x; k < cnt; k += kBuildBlock) {
-    const uint64_t u = u0 + k;
-    bvht_AABB b;
-    if (LEVEL0) {
-      const uint32_t p = (uint32_t)u;
-      uint32_t f = sorted[sd.faceBase + p] - sd.faceBase;
-      if (f >= n) f = 0;  // never read outside the mesh, even from a corrupt permutation
-      const bvht_Triangle t = faces[sd.faceBase + f];
-      b = triangleBox(vert[t.v1], vert[t.v2], vert[t.v3]);
+  if (LEVEL0) {
+    // Every sorted index, face and vertex gather of the thread's leaves is
+    // issued before any is used: one chain of dependent loads per thread
+    // instead of one per leaf.
+    constexpr int R = (T + kBuildBlock - 1) / kBuildBlock;
+    uint32_t f[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const uint32_t k = threadIdx.x + i * kBuildBlock;
+      f[i] = k < cnt ? sorted[sd.faceBase + (uint32_t)(u0 + k)] - sd.faceBase : 0u;
+      if (f[i] >= n) f[i] = 0;  // never read outside the mesh, even from a corrupt permutation
+    }
+    bvht_Triangle t[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) t[i] = faces[sd.faceBase + f[i]];
+    bvht_vec3 va[R], vb[R], vc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      va[i] = vert[t[i].v1];
+      vb[i] = vert[t[i].v2];
+      vc[i] = vert[t[i].v3];
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const uint32_t k = threadIdx.x + i * kBuildBlock;
+      if (k >= cnt) continue;
+      const uint32_t p = (uint32_t)(u0 + k);
+      const bvht_AABB b = triangleBox(va[i], vb[i], vc[i]);
       uint32_t parent, isLeft;
       parentOf(p, p, N1, parent, isLeft);
       // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
-      storeNode(nodes + N1 + p, b, parent, f, 1u, 1u, isLeft, 255u, surfaceArea(b));
-    } else {
+      storeNode(nodes + N1 + p, b, parent, f[i], 1u, 1u, isLeft, 255u, surfaceArea(b));
+      s_box[0][k] = b.minMax[0].x;
+      s_box[1][k] = b.minMax[0].y;
+      s_box[2][k] = b.minMax[0].z;
+      s_box[3][k] = b.minMax[1].x;
+      s_box[4][k] = b.minMax[1].y;
+      s_box[5][k] = b.minMax[1].z;
+    }
+  }
+  for (uint32_t k = threadIdx.x; !LEVEL0 && k < cnt; k += kBuildBlock) {
+    const uint64_t u = u0 + k;
+    bvht_AABB b;
+    {
       const uint32_t l = (uint32_t)(u << unitLog);
       const uint32_t r = (uint32_t)min((u + 1) << unitLog, (uint64_t)n) - 1;
       const float *src = reinterpret_cast<const float *>(nodes + rangeNodeIndex(l, r, N1));
@@ -112,23 +142,36 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__rest
   }
   __syncthreads();
 
+  // Level j merges entries 2a and 2a + 1 of level j - 1 into entry a (a
+  // compact layout: lanes read with stride 2 and write contiguously, where
+  // an in-place tree reduction would read with stride 2^j and serialise on
+  // LDS banks).  Reads go to registers before a barrier, then the writes.
+  constexpr int RH = (T / 2 + kBuildBlock - 1) / kBuildBlock;
   for (uint32_t j = 1; j <= (uint32_t)TLOG; ++j) {
     const uint32_t half = 1u << (j - 1);
     const uint32_t nb = (cnt + (1u << j) - 1) >> j;
-    for (uint32_t a = threadIdx.x; a < nb; a += kBuildBlock) {
-      const uint32_t k0 = a << j, km = k0 + half;
-      if (km >= cnt) continue;  // right half empty: same range as the left half
+    const uint32_t nPrev = (cnt + half - 1) >> (j - 1);
+    float res[RH][6];
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      const uint32_t a = threadIdx.x + q * kBuildBlock;
+      if (a >= nb) continue;
+      const uint32_t c0 = 2 * a, c1 = c0 + 1;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) res[q][c] = s_box[c][c0];
+      if (c1 >= nPrev) continue;  // right half empty: same range as the left half
       bvht_AABB b;
-      b.minMax[0] = {mergeMin(s_box[0][k0], s_box[0][km]), mergeMin(s_box[1][k0], s_box[1][km]),
-                     mergeMin(s_box[2][k0], s_box[2][km])};
-      b.minMax[1] = {mergeMax(s_box[3][k0], s_box[3][km]), mergeMax(s_box[4][k0], s_box[4][km]),
-                     mergeMax(s_box[5][k0], s_box[5][km])};
-      s_box[0][k0] = b.minMax[0].x;
-      s_box[1][k0] = b.minMax[0].y;
-      s_box[2][k0] = b.minMax[0].z;
-      s_box[3][k0] = b.minMax[1].x;
-      s_box[4][k0] = b.minMax[1].y;
-      s_box[5][k0] = b.minMax[1].z;
+      b.minMax[0] = {mergeMin(res[q][0], s_box[0][c1]), mergeMin(res[q][1], s_box[1][c1]),
+                     mergeMin(res[q][2], s_box[2][c1])};
+      b.minMax[1] = {mergeMax(res[q][3], s_box[3][c1]), mergeMax(res[q][4], s_box[4][c1]),
+                     mergeMax(res[q][5], s_box[5][c1])};
+      res[q][0] = b.minMax[0].x;
+      res[q][1] = b.minMax[0].y;
+      res[q][2] = b.minMax[0].z;
+      res[q][3] = b.minMax[1].x;
+      res[q][4] = b.minMax[1].y;
+      res[q][5] = b.minMax[1].z;
+      const uint32_t k0 = a << j, km = k0 + half;
       const uint32_t l = (uint32_t)((u0 + k0) << unitLog);
       const uint32_t m = (uint32_t)((u0 + km) << unitLog);
       const uint32_t r = (uint32_t)min((u0 + k0 + (1u << j)) << unitLog, (uint64_t)n) - 1;
@@ -136,6 +179,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__rest
       parentOf(l, r, N1, parent, isLeft);
       storeNode(nodes + rangeNodeIndex(l, r, N1), b, parent, rangeNodeIndex(l, m - 1, N1), rangeNodeIndex(m, r, N1), 0u,
                 isLeft, 0u, surfaceArea(b));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      const uint32_t a = threadIdx.x + q * kBuildBlock;
+      if (a >= nb) continue;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_box[c][a] = res[q][c];
     }
     __syncthreads();
   }

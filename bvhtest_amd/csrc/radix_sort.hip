@@ -69,8 +69,11 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
 
   // Stable in-wave ranking: item-major order inside a wave's segment equals
   // tile order.  Peers (same digit) are found with 8 ballots; the lowest
-  // peer bumps the wave's counter for the digit.
-  volatile uint32_t *cnt = &s_wave[wave][0];
+  // peer bumps the wave's counter for the digit.  The counters are read and
+  // written with relaxed workgroup-scope atomics: plain LDS instructions the
+  // compiler may not cache or reorder (a volatile pointer would become a
+  // generic one and compile to FLAT accesses that also wait on global loads).
+  uint32_t *cnt = &s_wave[wave][0];
   const uint64_t lt = __lanemask_lt();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
@@ -83,9 +86,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
     uint32_t lower = (uint32_t)__popcll(peers & lt);
-    uint32_t c = cnt[d];
+    uint32_t c = __hip_atomic_load(cnt + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     rank[i] = c + lower;
-    if (valid && lower == 0) cnt[d] = c + (uint32_t)__popcll(peers);
+    if (valid && lower == 0)
+      __hip_atomic_store(cnt + d, c + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __syncthreads();
 

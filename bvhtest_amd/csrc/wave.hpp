@@ -26,6 +26,8 @@ __device__ __forceinline__ uint32_t loadRelaxedAgent(const uint32_t *p) {
 __device__ __forceinline__ void fenceReleaseAgent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void fenceAcquireAgent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
+__device__ __forceinline__ uint32_t divUpDev(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
 // Inclusive scan across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t x) {
   const uint32_t lane = threadIdx.x & 63;

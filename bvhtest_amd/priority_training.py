@@ -1,8 +1,8 @@
 """Learned Bittner13 removal priority: records, training and evaluation.
 
 Records come from the plain GPU optimizer itself (1% of the nodes per step,
-32 chunks, the reference defaults) on seeded scenes of the C1 kind (icosphere
-clusters + triangle soup).  Every patch a step evaluates is recorded as the
+32 chunks, the reference defaults) on 32 seeded scenes of the C1 kind
+(icosphere clusters + triangle soup, ~1M triangles each).  Every patch a step evaluates is recorded as the
 features of its node at selection time and the SAH gain the searches found for
 it.  Each step also probes uniformly sampled inner nodes, so the model sees
 nodes the heuristic would not have picked.  The regression target is an
@@ -10,8 +10,9 @@ increasing map of the gain (z-score, asinh or rank; see transform_target):
 the optimizer only uses the order of the model's outputs.
 
 The model is trained with Adam (hand-written BF16-MFMA kernels, bvht_mlpTrain).
-Evaluation is time to reach the plain optimizer's final SAH(1.2, 1) within
-0.5% on held-out seeds. The learned run falls back to the plain heuristic for
+Evaluation is time to reach the plain optimizer's final SAH(1.2, 1) after 100
+steps within 0.5%, on 8 held-out ~16.9M-triangle scenes (8,192 icospheres +
+2M soup triangles). The learned run falls back to the plain heuristic for
 one step whenever a learned step fails to lower the true SAH.
 
     python -m bvhtest_amd.priority_training --out weights.npy
@@ -150,15 +151,15 @@ def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True, help="weights (.npy, fp32)")
-    ap.add_argument("--train-seeds", default="100-107")
-    ap.add_argument("--eval-seeds", default="200-201")
+    ap.add_argument("--train-seeds", default="100-131")
+    ap.add_argument("--eval-seeds", default="200-207")
     ap.add_argument("--spheres", type=int, default=512)
     ap.add_argument("--soup", type=int, default=125000)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--probes", type=int, default=20000)
-    ap.add_argument("--eval-spheres", type=int, default=2048)
-    ap.add_argument("--eval-soup", type=int, default=500000)
-    ap.add_argument("--eval-steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--probes", type=int, default=10000)
+    ap.add_argument("--eval-spheres", type=int, default=8192)
+    ap.add_argument("--eval-soup", type=int, default=2000000)
+    ap.add_argument("--eval-steps", type=int, default=100)
     ap.add_argument("--target", default="asinh", choices=("z", "asinh", "rank"))
     ap.add_argument("--save-records", help="write a random subsample of the records (.npz: x, g)")
     ap.add_argument("--save-count", type=int, default=200000)

@@ -188,7 +188,10 @@ typedef struct bvht_LBVH_Options {
                             sorted index, lib/builder/LBVH_CUDA.cu:205-212); 1: Karras 2012 over
                             the full 64-bit key (morton<<32 | index) */
   uint32_t tileLog2;     /* log2 of leaves per build workgroup (8..11), 0 = default */
-  uint32_t digitBits;    /* radix digit width of the sort (8 or 11... ), 0 = default */
+  uint32_t digitBits;    /* first sort digit: 8 (default, 0 = default) = onesweep LSD radix sort,
+                            8-bit digits; 16 = most-significant-digit split on the top 16 key
+                            bits, then in-LDS sorts of the buckets by (key, index).  Both give
+                            the permutation of the reference's stable sort. */
 } bvht_LBVH_Options;
 
 /* LBVH_WorkingMemory -- lib/builder/LBVH_CUDA.hpp:37-47 (scratch handles). */
@@ -237,9 +240,25 @@ void LBVH_doCUDASyc(void);
 /* Whole build on the stream without host round trips (extension: the scene
  * box stays on the device).  Same result as the five staged calls above. */
 bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh);
+/* Error bits of the last sort (extension, synchronises): 0 = the sort ran
+ * to completion.  MSD sort: 1 = histogram does not match the keys, 2 = a
+ * bucket ran out of key bits, 4 = a piece exceeded the in-LDS limit;
+ * onesweep: 1 = a look-back wait gave up.
+ * 0xFFFFFFFF on a bad argument or a failed copy. */
+uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem);
+/* Sorts the n (key, index) pairs of device keys[0..n) (keyBits 30: uint32
+ * keys, 63: uint64 keys, each below 2^keyBits) by key then index, i.e. the
+ * permutation of the reference's stable radix sort with index values
+ * (lib/builder/LBVH_CUDA.cu:185-193), with the LBVH build's sort (digitBits
+ * 8: onesweep, 16: MSD, 0: default = 8).  valsOut (device, n) receives the
+ * indices, keysOut (device, optional) the sorted keys.  Extension for
+ * testing the sort alone; allocates its own scratch and synchronises. */
+bool bvht_sortKeyIndexPairs(const void *keys, uint32_t n, uint32_t keyBits, uint32_t digitBits, void *keysOut,
+                            uint32_t *valsOut);
 /* Per-kernel device times (ms, HIP events on the library stream) of the last
  * bvht_LBVH_build when timing is on; bvht_LBVH_stageName(i) names kernel i
- * (k_sceneBox, k_morton, k_onesweep per pass, k_buildLevel0, k_buildLevelUp).
+ * (k_sceneBox, k_morton or k_mortonMsd, the sort's kernels, k_buildLevel0,
+ * k_buildLevelUp).
  * Returns the number of entries written. */
 void bvht_LBVH_setTiming(bool enable);
 int bvht_LBVH_stageTimes(float *ms, int maxStages);

@@ -42,19 +42,21 @@ def nodes_equal(a, b):
     return False
 
 
+@pytest.mark.parametrize("digits", [16, 8], ids=["msd", "onesweep"])
 @pytest.mark.parametrize("bits", [30, 63])
 @pytest.mark.parametrize("name", ["soup1", "soup2", "soup3", "soup17", "soup64", "soup1000", "soup1025", "soup4097",
                                   "soup100003", "heightfield65536", "spheres_small", "flat_dupes"])
-def test_fused_build_bit_exact(bvh, orc, cases, name, bits):
+def test_fused_build_bit_exact(bvh, orc, cases, name, bits, digits):
     v, f = cases[name]
-    print(f"case {name} n={len(f)} bits={bits}", flush=True)
+    print(f"case {name} n={len(f)} bits={bits} digits={digits}", flush=True)
     keys, sf, expect, scene = orc.lbvh(v, f, bits)
     mesh = bvh.DeviceMesh(v, f)
-    b = bvh.LBVHBuilder(mesh, bits)
+    b = bvh.LBVHBuilder(mesh, bits, digit_bits=digits)
     tree = b.allocate_bvh()
     b.build(tree)
     got_sf = b.sorted_faces()
     got = tree.download()
+    assert b.sort_status() == 0
     assert np.array_equal(got_sf, sf)
     assert nodes_equal(got, expect)
 
@@ -80,13 +82,14 @@ def test_staged_build_bit_exact(bvh, orc, cases, name, bits):
 
 @pytest.mark.parametrize("bits", [30, 63])
 @pytest.mark.parametrize("name", ["soup17", "soup4097", "heightfield65536", "spheres_small", "flat_dupes"])
-def test_karras_topology_extension(bvh, orc, cases, name, bits):
+@pytest.mark.parametrize("digits", [16, 8], ids=["msd", "onesweep"])
+def test_karras_topology_extension(bvh, orc, cases, name, bits, digits):
     """topology 1: Karras 2012 over the full (morton, index) key."""
     v, f = cases[name]
-    print(f"case {name} n={len(f)} bits={bits}", flush=True)
+    print(f"case {name} n={len(f)} bits={bits} digits={digits}", flush=True)
     keys, sf, expect, _ = orc.lbvh(v, f, bits, topology=1)
     mesh = bvh.DeviceMesh(v, f)
-    b = bvh.LBVHBuilder(mesh, bits, topology=1)
+    b = bvh.LBVHBuilder(mesh, bits, topology=1, digit_bits=digits)
     tree = b.allocate_bvh()
     b.build(tree)
     assert np.array_equal(b.sorted_keys().astype(np.uint64), keys)
@@ -113,12 +116,15 @@ def test_full_size_63bit_scene(bvh, orc):
     print(f"full scene n={len(f)}", flush=True)
     keys, sf, expect, _ = orc.lbvh(v, f, 63)
     mesh = bvh.DeviceMesh(v, f)
-    b = bvh.LBVHBuilder(mesh, 63)
-    tree = b.allocate_bvh()
-    for _ in range(2):  # a rebuild into the same memory gives the same tree
-        b.build(tree)
-        assert np.array_equal(b.sorted_faces(), sf)
-        assert tree.download().tobytes() == expect.tobytes()
+    for digits in (16, 8):
+        b = bvh.LBVHBuilder(mesh, 63, digit_bits=digits)
+        tree = b.allocate_bvh()
+        for _ in range(2):  # a rebuild into the same memory gives the same tree
+            b.build(tree)
+            assert b.sort_status() == 0
+            assert np.array_equal(b.sorted_faces(), sf)
+            assert tree.download().tobytes() == expect.tobytes()
+        tree.free()
 
 
 def test_errors_are_reported(bvh):
