@@ -257,8 +257,9 @@ def run_bench(args):
 
 def batch_bench(bt, L, reps=5):
     """512 meshes of log-uniform size in [1K, 500K] triangles: one segmented
-    build replayed from a hipGraph (tile size autotuned) vs. 512 sequential
-    single-mesh builds of the same meshes (30-bit keys both)."""
+    build replayed from a hipGraph (tile size and sort digit width
+    autotuned) vs. 512 sequential single-mesh builds of the same meshes
+    (30-bit keys both)."""
     t0 = time.time()
     v, f, fc, vc = bt.scene_batch(512, 1000, 1000, 500000)
     total = int(fc.sum())
@@ -294,8 +295,10 @@ def batch_bench(bt, L, reps=5):
     seq_ms = (time.perf_counter() - t0) * 1e3 / reps
     for lb in builders:
         lb.free()
-    out = {"meshes": int(len(fc)), "triangles": total, "tile_log2": int(best),
-           "autotune_ms": [round(x, 3) for x in ms], "graph_ms": round(graph_ms, 3),
+    out = {"meshes": int(len(fc)), "triangles": total, "tile_log2": int(best), "radix_bits": int(b.mem.radixBits),
+           "autotune_ms": {"radix8_tiles_8_to_11": [round(x, 3) for x in ms[:4]],
+                           "radix10_tiles_8_to_11": [round(x, 3) for x in ms[4:]]},
+           "graph_ms": round(graph_ms, 3),
            "graph_mtri_per_s": round(total / graph_ms / 1e3, 1), "sequential_ms": round(seq_ms, 3),
            "sequential_mtri_per_s": round(total / seq_ms / 1e3, 1), "speedup": round(seq_ms / graph_ms, 2)}
     print(f"# batched: {out}", file=sys.stderr, flush=True)

@@ -592,7 +592,7 @@ def scene_batch(num_meshes=512, seed=1000, min_tris=1000, max_tris=500000):
 class BatchBuilder:
     """Segmented LBVH build of concatenated meshes (one launch sequence, optional hipGraph)."""
 
-    def __init__(self, vert, faces, face_counts, vert_counts, tile_log2=10):
+    def __init__(self, vert, faces, face_counts, vert_counts, tile_log2=10, radix_bits=8):
         L = lib()
         fc = np.ascontiguousarray(face_counts, np.uint32)
         vc = np.ascontiguousarray(vert_counts, np.uint32)
@@ -600,6 +600,7 @@ class BatchBuilder:
                                              vc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), tile_log2)
         if not self.mem.lRes:
             _fail("bvht_LBVH_allocateBatch")
+        self.mem.radixBits = radix_bits
         self.face_counts, self.vert_counts = fc, vc
         self.vb = DeviceBuffer(vert.nbytes).upload(np.ascontiguousarray(vert, np.float32))
         self.fb = DeviceBuffer(faces.nbytes).upload(np.ascontiguousarray(faces, np.uint32))
@@ -610,7 +611,9 @@ class BatchBuilder:
             _fail("bvht_LBVH_buildBatch")
 
     def autotune(self):
-        ms = (ctypes.c_float * 4)()
+        """(tile log2, per-choice mean ms: tiles 2^8..2^11 with 8-bit digits, then with 10-bit digits);
+        the chosen digit width is in self.mem.radixBits."""
+        ms = (ctypes.c_float * 8)()
         best = lib().bvht_LBVH_autotuneBatch(ctypes.byref(self.mem), self.vb.ptr, self.fb.ptr, self.nb.ptr, ms)
         return best, list(ms)
 

@@ -48,7 +48,7 @@ class LBVHWorkingMemory(ctypes.Structure):
 
 class LBVHBatchWorkingMemory(ctypes.Structure):
     _fields_ = [("lRes", c_bool), ("numMeshes", c_u32), ("totalFaces", c_u32), ("totalVert", c_u32),
-                ("totalNodes", c_u32), ("tileLog2", c_u32), ("internal", c_vp)]
+                ("totalNodes", c_u32), ("tileLog2", c_u32), ("radixBits", c_u32), ("internal", c_vp)]
 
 
 class AlgoCFG(ctypes.Structure):

@@ -6,10 +6,14 @@
 // lib/builder/LBVH_CUDA.cu applied per mesh.  Stages:
 //   k_segBox     per tile of one mesh: triangle boxes, block reduce, 6 atomic
 //                min/max (order-preserving u32 encoding) into the mesh box;
-//   k_segMorton  per tile: the mesh's offset/scale (as LBVH_CUDA.cu:171-180),
-//                key = mesh id << 30 | 30-bit Morton code, sort histograms;
-//   onesweep     one stable sort over all meshes (segment id in the high
-//                bits keeps segments apart, stability keeps face order);
+//   k_segMorton  per chunk of one mesh: the mesh's offset/scale (as
+//                LBVH_CUDA.cu:171-180), the 30-bit Morton code, and the
+//                mesh's digit histograms of every sort pass;
+//   k_segPrefix  per (mesh, pass): exclusive scan of the histogram;
+//   k_segOnesweep one segmented onesweep pass: sort tiles never cross a
+//                mesh, the look-back stops at the mesh's first tile, and
+//                digit offsets start at the mesh's prefix; u32 keys, 8- or
+//                10-bit digits (4 or 3 passes, chosen by the autotuner);
 //   k_buildLevel the tile hierarchy per mesh (tiles never cross meshes).
 #include "lbvh_kernels.hpp"
 
@@ -65,40 +69,236 @@ __global__ __launch_bounds__(kBuildBlock) void k_segBox(const bvht_vec3 *__restr
   }
 }
 
-template <int TLOG>
+constexpr int kSegItems = 15;                               // keys per thread of a sort tile
+constexpr uint32_t kSegTile = kSortBlock * kSegItems;       // pairs per sort tile
+constexpr int kChunkLog = 15;                               // triangles per Morton workgroup
+constexpr int kSegMaxHist = 3 * 1024;                       // passes x radix, both digit widths
+constexpr uint32_t kSegFlagAgg = 1u << 30, kSegFlagInc = 2u << 30, kSegCountMask = (1u << 30) - 1;
+
+__host__ __device__ constexpr int segPasses(int radixBits) { return (30 + radixBits - 1) / radixBits; }
+
+// Morton keys of one chunk (up to 2^kChunkLog triangles of one mesh) and
+// the mesh's histograms of every pass (LDS, then one atomic per non-empty
+// bin).  Four triangles per thread per trip, loads issued before use.
+template <int RB>
 __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__restrict__ vertAll,
                                                            const bvht_Triangle *__restrict__ faces,
                                                            const SegDesc *__restrict__ segs,
-                                                           const uint2 *__restrict__ tiles,
+                                                           const uint2 *__restrict__ chunks,
                                                            const uint32_t *__restrict__ boxOrd,
-                                                           uint64_t *__restrict__ keys, uint32_t *__restrict__ hist,
-                                                           int passes) {
-  __shared__ uint32_t s_hist[8][kSortRadix];
-  for (int p = 0; p < passes; ++p) s_hist[p][threadIdx.x] = 0;
-  const uint2 t = tiles[blockIdx.x];
-  const SegDesc sd = segs[t.x];
+                                                           uint32_t *__restrict__ keys, uint32_t *__restrict__ segHist) {
+  constexpr int R = 1 << RB, P = segPasses(RB);
+  static_assert(P * R <= kSegMaxHist, "histogram fits the LDS array");
+  __shared__ uint32_t s_hist[P * R];
+  for (int i = threadIdx.x; i < P * R; i += kBuildBlock) s_hist[i] = 0;
+  const uint2 c = chunks[blockIdx.x];
+  const SegDesc sd = segs[c.x];
   bvht_vec3 off, sc;
   {
     float m[6];
-    for (int a = 0; a < 6; ++a) m[a] = o2f(boxOrd[6 * t.x + a]);
+    for (int a = 0; a < 6; ++a) m[a] = o2f(boxOrd[6 * c.x + a]);
     // LBVH_calcMortonCodes host part (LBVH_CUDA.cu:171-180), f32, IEEE division
     off = {0.0f - m[0], 0.0f - m[1], 0.0f - m[2]};
     sc = {__fdiv_rn(1.0f, m[3] + off.x), __fdiv_rn(1.0f, m[4] + off.y), __fdiv_rn(1.0f, m[5] + off.z)};
   }
   __syncthreads();
   const bvht_vec3 *vert = vertAll + sd.vertBase;
-  const uint32_t p0 = t.y << TLOG, p1 = min(p0 + (1u << TLOG), sd.n);
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBuildBlock) {
-    const bvht_Triangle tri = faces[sd.faceBase + p];
-    const bvht_vec3 c = centroid(vert[tri.v1], vert[tri.v2], vert[tri.v3]);
-    const uint64_t key = ((uint64_t)t.x << 30) | (uint64_t)morton30(c, off, sc);
-    keys[sd.faceBase + p] = key;
-    for (int q = 0; q < passes; ++q) atomicAdd(&s_hist[q][(uint32_t)(key >> (q * kSortRadixBits)) & (kSortRadix - 1)], 1u);
+  const bvht_Triangle *fc = faces + sd.faceBase;
+  const uint32_t p0 = c.y << kChunkLog, p1 = min(p0 + (1u << kChunkLog), sd.n);
+  constexpr int U = 4;
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += U * kBuildBlock) {
+    bvht_Triangle tri[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t q = p + u * kBuildBlock;
+      tri[u] = fc[q < p1 ? q : p];
+    }
+    bvht_vec3 va[U], vb[U], vc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      va[u] = vert[tri[u].v1];
+      vb[u] = vert[tri[u].v2];
+      vc[u] = vert[tri[u].v3];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t q = p + u * kBuildBlock;
+      if (q >= p1) continue;
+      const uint32_t key = morton30(centroid(va[u], vb[u], vc[u]), off, sc);
+      keys[sd.faceBase + q] = key;
+#pragma unroll
+      for (int pass = 0; pass < P; ++pass) atomicAdd(&s_hist[pass * R + ((key >> (pass * RB)) & (R - 1))], 1u);
+    }
   }
   __syncthreads();
-  for (int q = 0; q < passes; ++q) {
-    const uint32_t c = s_hist[q][threadIdx.x];
-    if (c) atomicAdd(hist + q * kSortRadix + threadIdx.x, c);
+  uint32_t *h = segHist + (size_t)c.x * (P * R);
+  for (int i = threadIdx.x; i < P * R; i += kBuildBlock) {
+    const uint32_t n = s_hist[i];
+    if (n) atomicAdd(h + i, n);
+  }
+}
+
+// Exclusive scan of one (mesh, pass) histogram row, in place.
+template <int RB>
+__global__ __launch_bounds__(kSortBlock) void k_segPrefix(uint32_t *__restrict__ segHist) {
+  constexpr int R = 1 << RB, D = R / kSortBlock;
+  __shared__ uint32_t s_scan[kSortBlock / 64];
+  uint32_t *row = segHist + (size_t)blockIdx.x * R + threadIdx.x * D;
+  uint32_t v[D], sum = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    v[j] = row[j];
+    sum += v[j];
+  }
+  uint32_t run = blockExclusiveScan<kSortBlock>(sum, s_scan);
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    row[j] = run;
+    run += v[j];
+  }
+}
+
+// One pass of the segmented onesweep: stable by (mesh, digit).  Sort tiles
+// (tileTab: mesh, tile within the mesh) are taken in order from `counter`;
+// a tile ranks its keys with RB wave ballots, publishes per-digit counts
+// (flag | count in one u32), looks back only over earlier tiles of its own
+// mesh, and scatters through LDS.  Thread t owns digits [t D, t D + D).
+template <int RB>
+__global__ __launch_bounds__(kSortBlock) void k_segOnesweep(const uint32_t *__restrict__ keysIn,
+                                                            const uint32_t *__restrict__ valsIn,
+                                                            uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
+                                                            const uint2 *__restrict__ tileTab,
+                                                            const SegDesc *__restrict__ segs, uint32_t shift,
+                                                            const uint32_t *__restrict__ prefix, int passes,
+                                                            uint32_t *__restrict__ status, uint32_t *__restrict__ counter,
+                                                            uint32_t *__restrict__ err) {
+  constexpr int R = 1 << RB, D = R / kSortBlock, WAVES = kSortBlock / 64, TILE = (int)kSegTile;
+  __shared__ uint32_t s_wave[WAVES][R];
+  __shared__ uint32_t s_bstart[R], s_gstart[R];
+  __shared__ uint32_t s_scan[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_keys[TILE], s_vals[TILE];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  for (int i = tid; i < WAVES * R; i += kSortBlock) (&s_wave[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t g = s_tile;
+  const uint2 tt = tileTab[g];
+  const SegDesc sd = segs[tt.x];
+  const uint32_t lt = tt.y, segStart = sd.faceBase, segN = sd.n;
+  const uint32_t base = lt * TILE + wave * 64 * kSegItems;
+
+  uint32_t key[kSegItems], val[kSegItems], rank[kSegItems];
+#pragma unroll
+  for (int i = 0; i < kSegItems; ++i) {
+    const uint32_t idx = base + i * 64 + lane;
+    const bool valid = idx < segN;
+    key[i] = valid ? keysIn[segStart + idx] : 0u;
+    val[i] = valid ? (valsIn ? valsIn[segStart + idx] : segStart + idx) : 0u;
+  }
+  // stable in-wave ranking (see k_onesweep): item-major order inside a
+  // wave's segment is tile order
+  uint32_t *cnt = &s_wave[wave][0];
+  const uint64_t lt64 = __lanemask_lt();
+#pragma unroll
+  for (int i = 0; i < kSegItems; ++i) {
+    const bool valid = base + i * 64 + lane < segN;
+    const uint32_t d = (key[i] >> shift) & (R - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t lower = (uint32_t)__popcll(peers & lt64);
+    const uint32_t c = __hip_atomic_load(cnt + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    rank[i] = c + lower;
+    if (valid && lower == 0)
+      __hip_atomic_store(cnt + d, c + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+
+  // per owned digit: per-wave exclusive offsets and the tile total
+  uint32_t tot[D], sum = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const uint32_t d = tid * D + j;
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_wave[w][d];
+      s_wave[w][d] = run;
+      run += c;
+    }
+    tot[j] = run;
+    sum += run;
+  }
+  uint32_t *st = status + (size_t)g * R + tid * D;
+#pragma unroll
+  for (int j = 0; j < D; ++j) storeRelaxedAgent(st + j, (lt == 0 ? kSegFlagInc : kSegFlagAgg) | tot[j]);
+  const uint32_t bex = blockExclusiveScan<kSortBlock>(sum, s_scan);
+  uint32_t excl[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) excl[j] = 0;
+  if (lt > 0) {
+    // walk back over the mesh's earlier tiles; its first tile publishes an
+    // inclusive count at once, so every walk ends at or before it
+    uint32_t t = g - 1, spins = 0, doneMask = 0;
+    while (doneMask != (1u << D) - 1) {
+      uint32_t w[D];
+      bool ready = true;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        w[j] = loadRelaxedAgent(status + (size_t)t * R + tid * D + j);
+        if (!(doneMask & (1u << j)) && (w[j] & ~kSegCountMask) == 0) ready = false;
+      }
+      if (!ready) {
+        if (++spins > (1u << 24)) {  // a predecessor that never publishes: error flag instead of a hang
+          atomicOr(err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        if (doneMask & (1u << j)) continue;
+        excl[j] += w[j] & kSegCountMask;
+        if ((w[j] & ~kSegCountMask) == kSegFlagInc) doneMask |= 1u << j;
+      }
+      --t;
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) storeRelaxedAgent(st + j, kSegFlagInc | (excl[j] + tot[j]));
+  }
+  const uint32_t *pre = prefix + (size_t)tt.x * passes * R + tid * D;
+  uint32_t run = bex;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    s_bstart[tid * D + j] = run;
+    s_gstart[tid * D + j] = segStart + pre[j] + excl[j];
+    run += tot[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSegItems; ++i) {
+    if (base + i * 64 + lane < segN) {
+      const uint32_t d = (key[i] >> shift) & (R - 1);
+      const uint32_t pos = s_bstart[d] + s_wave[wave][d] + rank[i];
+      s_keys[pos] = key[i];
+      s_vals[pos] = val[i];
+    }
+  }
+  __syncthreads();
+  const uint32_t tileCount = min((uint32_t)TILE, segN - lt * TILE);
+  const uint32_t segEnd = segStart + segN;
+  for (uint32_t j = tid; j < tileCount; j += kSortBlock) {
+    const uint32_t k = s_keys[j];
+    const uint32_t d = (k >> shift) & (R - 1);
+    const uint32_t out = s_gstart[d] + (j - s_bstart[d]);
+    if (out >= segEnd) continue;  // only reachable after a bounded-wait error
+    if (keysOut) keysOut[out] = k;
+    valsOut[out] = s_vals[j];
   }
 }
 
@@ -110,29 +310,62 @@ struct BatchInternal {
   std::vector<uint2 *> tiles;
   std::vector<uint32_t> tileCounts;
   uint32_t *boxOrd = nullptr;
-  uint64_t *keys[2] = {nullptr, nullptr};
+  uint32_t *keys[2] = {nullptr, nullptr};
   uint32_t *vals[2] = {nullptr, nullptr};
+  int radixBits = 8;
+  uint2 *chunks = nullptr;     // (mesh, chunk) of the Morton pass
+  uint32_t numChunks = 0;
+  uint2 *sortTiles = nullptr;  // (mesh, tile) of the sort passes
+  uint32_t numSortTiles = 0;
+  // sort scratch: [0, 16) pass counters + error word, then the mesh
+  // histograms / prefixes (M x passes x radix), then the look-back status
+  // (passes x sort tiles x radix); all of it is zeroed before every build
   uint32_t *sortScratch = nullptr;
-  SortPlan plan;
+  size_t histWords = 0, statusWords = 0;
   hipStream_t capStream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void *capVert = nullptr, *capFaces = nullptr, *capNodes = nullptr;
 };
 
+template <int RB>
+bool segSort(BatchInternal *in, int &result) {
+  constexpr int R = 1 << RB, P = segPasses(RB);
+  uint32_t *counters = in->sortScratch, *hist = in->sortScratch + 16;
+  uint32_t *status = hist + (size_t)in->M * P * R;
+  k_segPrefix<RB><<<in->M * P, kSortBlock, 0, stream()>>>(hist);
+  if (!checkLaunch("k_segPrefix")) return false;
+  int cur = 0;
+  for (int pass = 0; pass < P; ++pass) {
+    const bool last = pass == P - 1;
+    k_segOnesweep<RB><<<in->numSortTiles, kSortBlock, 0, stream()>>>(
+        in->keys[cur], pass == 0 ? nullptr : in->vals[cur], last ? nullptr : in->keys[cur ^ 1], in->vals[cur ^ 1],
+        in->sortTiles, in->segs, (uint32_t)(pass * RB), hist + (size_t)pass * R, P,
+        status + (size_t)pass * in->numSortTiles * R, counters + pass, counters + 15);
+    if (!checkLaunch("k_segOnesweep")) return false;
+    cur ^= 1;
+  }
+  result = cur;
+  return true;
+}
+
 template <int TLOG>
 bool batchStagesT(BatchInternal *in, const bvht_vec3 *vert, const bvht_Triangle *faces, bvht_BVHNode *nodes) {
   if (!fillU32(in->boxOrd, kOrdZero, 6ull * in->M)) return false;
-  if (!sortReset(in->plan, in->sortScratch)) return false;
+  const int P = segPasses(in->radixBits), R = 1 << in->radixBits;
+  if (!fillU32(in->sortScratch, 0u, 16 + (size_t)in->M * P * R + (size_t)P * in->numSortTiles * R)) return false;
   k_segBox<TLOG><<<in->tileCounts[0], kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->tiles[0], in->boxOrd);
   if (!checkLaunch("k_segBox")) return false;
-  k_segMorton<TLOG><<<in->tileCounts[0], kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->tiles[0], in->boxOrd,
-                                                                     in->keys[0], sortHist(in->plan, in->sortScratch),
-                                                                     in->plan.passes);
-  if (!checkLaunch("k_segMorton")) return false;
-  void *keys[2] = {in->keys[0], in->keys[1]};
-  const int r = sortRun(in->plan, keys, in->vals, true, false, in->sortScratch);
-  if (r < 0) return false;
+  int r = 0;
+  if (in->radixBits == 10) {
+    k_segMorton<10><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
+                                                                 in->keys[0], in->sortScratch + 16);
+    if (!checkLaunch("k_segMorton") || !segSort<10>(in, r)) return false;
+  } else {
+    k_segMorton<8><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
+                                                                in->keys[0], in->sortScratch + 16);
+    if (!checkLaunch("k_segMorton") || !segSort<8>(in, r)) return false;
+  }
   return launchTreeLevelsBatch(TLOG, nodes, in->vals[r], faces, vert, in->segs, in->tiles.data(),
                                in->tileCounts.data(), (int)in->tiles.size());
 }
@@ -229,7 +462,7 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
   in->tlog = (int)tileLog2;
   uint64_t fo = 0, vo = 0, no = 0;
   for (uint32_t i = 0; i < numMeshes; ++i) {
-    if (faceCounts[i] == 0 || faceCounts[i] > (1u << 30)) {
+    if (faceCounts[i] == 0 || faceCounts[i] >= (1u << 30)) {  // sort counts carry 30 bits
       setError("bvht_LBVH_allocateBatch: mesh %u has %u faces", i, faceCounts[i]);
       bvht_LBVH_freeBatch(&m);
       return m;
@@ -251,16 +484,31 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
     bvht_LBVH_freeBatch(&m);
     return m;
   }
-  int segBits = 1;
-  while ((1u << segBits) < numMeshes) segBits++;
-  in->plan = makeSortPlan(in->totalFaces, 8, 30 + segBits);
-  if (!devAlloc(&in->segs, numMeshes, "batch segments") ||
+  // Morton chunks and sort tiles: per mesh, never crossing one
+  std::vector<uint2> chunks, sortTiles;
+  for (uint32_t i = 0; i < numMeshes; ++i) {
+    const uint32_t n = faceCounts[i];
+    for (uint32_t c = 0; (uint64_t)c << kChunkLog < n; ++c) chunks.push_back(make_uint2(i, c));
+    for (uint32_t t = 0; (uint64_t)t * kSegTile < n; ++t) sortTiles.push_back(make_uint2(i, t));
+  }
+  in->numChunks = (uint32_t)chunks.size();
+  in->numSortTiles = (uint32_t)sortTiles.size();
+  // scratch for the larger of the two digit widths (10 bits: 3 passes x 1024)
+  in->histWords = (size_t)numMeshes * kSegMaxHist;
+  in->statusWords = (size_t)3 * in->numSortTiles * 1024;
+  static_assert(segPasses(8) * 256 <= kSegMaxHist && segPasses(10) * 1024 <= kSegMaxHist, "histogram rows");
+  if (!devAlloc(&in->chunks, chunks.size(), "batch chunks") ||
+      !check(hipMemcpy(in->chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice), "batch chunks") ||
+      !devAlloc(&in->sortTiles, sortTiles.size(), "batch sort tiles") ||
+      !check(hipMemcpy(in->sortTiles, sortTiles.data(), sizeof(uint2) * sortTiles.size(), hipMemcpyHostToDevice),
+             "batch sort tiles") ||
+      !devAlloc(&in->segs, numMeshes, "batch segments") ||
       !check(hipMemcpy(in->segs, in->segsHost.data(), sizeof(SegDesc) * numMeshes, hipMemcpyHostToDevice),
              "batch segments") ||
       !devAlloc(&in->boxOrd, 6ull * numMeshes, "batch boxes") || !devAlloc(&in->keys[0], in->totalFaces, "batch keys") ||
       !devAlloc(&in->keys[1], in->totalFaces, "batch keys") || !devAlloc(&in->vals[0], in->totalFaces, "batch values") ||
       !devAlloc(&in->vals[1], in->totalFaces, "batch values") ||
-      !devMalloc(reinterpret_cast<void **>(&in->sortScratch), sortScratchBytes(in->plan), "batch sort") ||
+      !devAlloc(&in->sortScratch, 16 + in->histWords + in->statusWords, "batch sort") ||
       !check(hipStreamCreateWithFlags(&in->capStream, hipStreamNonBlocking), "batch capture stream")) {
     bvht_LBVH_freeBatch(&m);
     return m;
@@ -270,6 +518,7 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
   m.totalVert = in->totalVert;
   m.totalNodes = in->totalNodes;
   m.tileLog2 = (uint32_t)in->tlog;
+  m.radixBits = (uint32_t)in->radixBits;
   m.lRes = true;
   return m;
 }
@@ -292,6 +541,14 @@ bool bvht_LBVH_buildBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *ve
     }
     dropGraph(in);
     if (!buildTiles(in, (int)mem->tileLog2)) return false;  // tables are per tile size
+  }
+  if (mem->radixBits != (uint32_t)in->radixBits) {
+    if (mem->radixBits != 8 && mem->radixBits != 10) {
+      setError("bvht_LBVH_buildBatch: radixBits must be 8 or 10");
+      return false;
+    }
+    dropGraph(in);
+    in->radixBits = (int)mem->radixBits;
   }
   if (!useGraph) return batchStages(in, vert, faces, nodes);
   if (in->exec && (in->capVert != vert || in->capFaces != faces || in->capNodes != nodes)) dropGraph(in);
@@ -326,6 +583,8 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem) {
     dropGraph(in);
     if (in->capStream) (void)hipStreamDestroy(in->capStream);
     freeTiles(in);
+    devFree(in->chunks);
+    devFree(in->sortTiles);
     devFree(in->segs);
     devFree(in->boxOrd);
     devFree(in->keys[0]);
@@ -414,18 +673,23 @@ int bvht_LBVH_autotuneBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *
     setError("bvht_LBVH_autotuneBatch: working memory not allocated");
     return 0;
   }
-  int best = (int)mem->tileLog2;
+  int best = (int)mem->tileLog2, bestBits = (int)mem->radixBits;
   float bestMs = 1e30f;
-  for (int t = 8; t <= 11; ++t) {
-    mem->tileLog2 = (uint32_t)t;
-    const float ms = timeRuns([&] { return bvht_LBVH_buildBatch(mem, vert, faces, nodes, false); }, 2);
-    if (msOut) msOut[t - 8] = ms;
-    if (ms > 0 && ms < bestMs) {
-      bestMs = ms;
-      best = t;
+  for (int bits : {8, 10}) {
+    for (int t = 8; t <= 11; ++t) {
+      mem->tileLog2 = (uint32_t)t;
+      mem->radixBits = (uint32_t)bits;
+      const float ms = timeRuns([&] { return bvht_LBVH_buildBatch(mem, vert, faces, nodes, false); }, 2);
+      if (msOut) msOut[(bits == 10 ? 4 : 0) + t - 8] = ms;
+      if (ms > 0 && ms < bestMs) {
+        bestMs = ms;
+        best = t;
+        bestBits = bits;
+      }
     }
   }
   mem->tileLog2 = (uint32_t)best;
+  mem->radixBits = (uint32_t)bestBits;
   return best;
 }
 

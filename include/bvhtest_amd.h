@@ -284,6 +284,7 @@ typedef struct bvht_LBVH_BatchWorkingMemory {
   uint32_t totalVert;
   uint32_t totalNodes;
   uint32_t tileLog2;
+  uint32_t radixBits; /* digit width of the segmented sort: 8 (4 passes) or 10 (3 passes) */
   void *internal;
 } bvht_LBVH_BatchWorkingMemory;
 
@@ -296,6 +297,11 @@ bool bvht_LBVH_buildBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *ve
 void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem);
 bool bvht_LBVH_batchOffsets(const bvht_LBVH_BatchWorkingMemory *mem, uint32_t i, uint32_t *faceOffset,
                             uint32_t *vertOffset, uint32_t *nodeOffset);
+/* Autotuner of the batched build: times every tile size 2^8..2^11 with both
+ * digit widths (8 and 10 bits) on the caller's input and keeps the fastest
+ * pair in mem->tileLog2 / mem->radixBits (every choice gives the same trees).
+ * msOut (8 floats, optional) receives the mean times, 8-bit digits first.
+ * Returns the chosen tile log2. */
 int bvht_LBVH_autotuneBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *vert, const bvht_Triangle *faces,
                             bvht_BVHNode *nodes, float *msOut);
 

@@ -19,18 +19,20 @@ def check_segments(bvh, orc, b, v, f, fc, vc, which):
         assert got.tobytes() == expect.tobytes(), f"segment {i} (n={n}) differs"
 
 
+@pytest.mark.parametrize("bits", [8, 10])
 @pytest.mark.parametrize("tile", [8, 9, 10, 11])
-def test_batch_segments_bit_exact(bvh, orc, tile):
+def test_batch_segments_bit_exact(bvh, orc, tile, bits):
     v, f, fc, vc = bvh.scene_batch(24, 77, 1, 20000)
-    print(f"tile={tile} meshes={len(fc)} faces={len(f)} min={fc.min()} max={fc.max()}", flush=True)
-    b = bvh.BatchBuilder(v, f, fc, vc, tile_log2=tile)
+    print(f"tile={tile} bits={bits} meshes={len(fc)} faces={len(f)} min={fc.min()} max={fc.max()}", flush=True)
+    b = bvh.BatchBuilder(v, f, fc, vc, tile_log2=tile, radix_bits=bits)
     b.build(use_graph=False)
     check_segments(bvh, orc, b, v, f, fc, vc, range(len(fc)))
 
 
-def test_batch_graph_replays(bvh, orc):
+@pytest.mark.parametrize("bits", [8, 10])
+def test_batch_graph_replays(bvh, orc, bits):
     v, f, fc, vc = bvh.scene_batch(40, 5, 100, 60000)
-    b = bvh.BatchBuilder(v, f, fc, vc)
+    b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=bits)
     b.build(use_graph=False)
     eager = b.nodes()
     for rep in range(4):
@@ -46,7 +48,23 @@ def test_batch_autotune_and_512_meshes(bvh, orc):
     print(f"512 meshes, {len(f)} faces", flush=True)
     b = bvh.BatchBuilder(v, f, fc, vc)
     best, ms = b.autotune()
-    print("autotune", best, ms, flush=True)
-    assert 8 <= best <= 11 and all(x > 0 for x in ms)
+    print("autotune", best, b.mem.radixBits, ms, flush=True)
+    assert 8 <= best <= 11 and b.mem.radixBits in (8, 10) and len(ms) == 8 and all(x > 0 for x in ms)
     b.build(use_graph=True)
     check_segments(bvh, orc, b, v, f, fc, vc, [0, 1, 2, 100, 255, 511])
+
+
+def test_batch_single_large_mesh_matches_standalone(bvh, orc):
+    """One mesh of 300,000 soup triangles plus two tiny ones: sort tiles of a
+    mesh span many look-back steps; a 1-triangle mesh has one partial tile."""
+    v1, f1 = bvh.scene_soup(300000, 5)
+    v2, f2 = bvh.scene_soup(1, 6)
+    v3, f3 = bvh.scene_soup(3841, 7)  # one sort tile plus one pair
+    v = np.concatenate([v1, v2, v3]).astype(np.float32)
+    f = np.concatenate([f1, f2, f3]).astype(np.uint32)
+    fc = np.array([len(f1), len(f2), len(f3)], np.uint32)
+    vc = np.array([len(v1), len(v2), len(v3)], np.uint32)
+    for bits in (8, 10):
+        b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=bits)
+        b.build(use_graph=False)
+        check_segments(bvh, orc, b, v, f, fc, vc, [0, 1, 2])
