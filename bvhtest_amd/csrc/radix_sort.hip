@@ -9,6 +9,7 @@ namespace {
 constexpr uint32_t kFlagAgg = 1u << 30;  // tile aggregate published
 constexpr uint32_t kFlagInc = 2u << 30;  // inclusive prefix published
 constexpr uint32_t kCountMask = (1u << 30) - 1;
+constexpr uint32_t kLookWindow = 8;  // look-back statuses loaded per round trip
 
 // Histogram of all passes: LDS counters per block, then one global atomic
 // per (pass, digit) per block.
@@ -107,11 +108,29 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
 
   uint32_t excl = 0;
   if (tile > 0) {
-    uint32_t t = tile - 1, spins = 0;
-    while (true) {
-      uint32_t s = loadRelaxedAgent(status + (size_t)t * kSortRadix + tid);
-      uint32_t f = s & ~kCountMask;
-      if (f == 0) {
+    // Look back over a window of kLookWindow predecessors per round trip:
+    // their words are loaded together (newest first) and consumed in order
+    // up to the first one not yet published, where the next round resumes.
+    constexpr uint32_t W = kLookWindow;
+    uint32_t next = tile, spins = 0;
+    bool done = false;
+    while (!done) {
+      uint32_t sv[W];
+#pragma unroll
+      for (uint32_t w = 0; w < W; ++w)  // tile 0 publishes an inclusive count at once: nothing lies before it
+        sv[w] = next >= w + 1 ? loadRelaxedAgent(status + (size_t)(next - 1 - w) * kSortRadix + tid) : kFlagInc;
+      uint32_t used = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < W; ++w) {
+        if (done || used != w) continue;
+        const uint32_t f = sv[w] & ~kCountMask;
+        if (f == 0) continue;  // not yet published: resume here
+        excl += sv[w] & kCountMask;
+        used = w + 1;
+        if (f == kFlagInc) done = true;
+      }
+      next -= used;
+      if (!done && used < W) {
         // bounded wait: a predecessor that never publishes (impossible when
         // the tile counter starts at zero) ends the pass with an error flag
         // instead of a hang
@@ -120,11 +139,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += s & kCountMask;
-      if (f == kFlagInc) break;
-      --t;
     }
     storeRelaxedAgent(st, kFlagInc | (excl + total));
   }
