@@ -240,7 +240,7 @@ def run_bench(args):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(v, f, bits)
+        cpu = cpu_baseline(v, f, bits, tree.download())
 
     out = {"metric": METRIC, "value": round(mtri, 2), "unit": "Mtri/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -321,10 +321,13 @@ def learned_bench(bt, seed=200, plain_steps=100):
     return r
 
 
-def cpu_baseline(v, f, bits, seconds=12.0):
+def cpu_baseline(v, f, bits, nodes, seconds=12.0):
     """Oracle OpenMP LBVH timed on this host: repeated full builds of the same
     scene for about `seconds` with every host thread, then about a third of
-    that with one thread.  `value` is the all-thread rate."""
+    that with one thread.  `value` is the all-thread rate.  Beside it, on the
+    same tree: the oracle tracer (the reference CPUTracer's traversal, OpenMP
+    dynamic 512) on every 8th primary ray of the 1920x1080 frame, and one step
+    of the sequential Bittner13 (1% of the nodes, single-threaded search)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as orc
 
@@ -343,11 +346,27 @@ def cpu_baseline(v, f, bits, seconds=12.0):
     mtri = len(f) * reps / secs / 1e6
     print(f"# cpu baseline: {reps} builds in {secs:.2f}s on {threads} threads; {reps1} in {secs1:.2f}s on 1",
           file=sys.stderr, flush=True)
+    rays = orc.gen_rays(1920, 1080, CAM["pos"], CAM["look_at"], CAM["up"], CAM["fov"])
+    sample_rays = np.ascontiguousarray(rays[::8])
+    t0 = time.perf_counter()
+    orc.trace(sample_rays, nodes, 0, v, f)
+    trace_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.bittner_seq(nodes, 0, 1.0, 1)
+    bn_s = time.perf_counter() - t0
+    print(f"# cpu baseline: trace {len(sample_rays) / trace_s / 1e6:.2f} Mrays/s, sequential Bittner13 {bn_s:.2f} s/step",
+          file=sys.stderr, flush=True)
     return {"value": round(mtri, 3), "unit": "Mtri/s", "cores": threads, "kind": "port",
             "sample": f"{reps} full builds of the same {len(f)}-triangle scene ({bits}-bit keys, encode to refit), "
                       f"oracle OpenMP LBVH, {secs:.2f}s of build time",
             "single_thread": {"value": round(len(f) * reps1 / secs1 / 1e6, 3), "unit": "Mtri/s",
-                              "sample": f"{reps1} builds, {secs1:.2f}s"}}
+                              "sample": f"{reps1} builds, {secs1:.2f}s"},
+            "trace": {"value": round(len(sample_rays) / trace_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                      "sample": f"every 8th primary ray of the 1920x1080 frame ({len(sample_rays)} rays), "
+                                f"oracle tracer on the same tree, {trace_s:.2f}s"},
+            "bittner_seq": {"value": round(bn_s * 1e3, 1), "unit": "ms/step", "cores": 1,
+                            "sample": "one step of the sequential Bittner13 (1% of the nodes, selection by the "
+                                      "reference priority) on the same tree"}}
 
 
 def main():

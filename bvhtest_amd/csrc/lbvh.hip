@@ -144,17 +144,22 @@ __global__ __launch_bounds__(kBlock) void k_morton(const bvht_vec3 *__restrict__
   __syncthreads();
   const bvht_vec3 off = {prm->offset[0], prm->offset[1], prm->offset[2]};
   const bvht_vec3 sc = {prm->scale[0], prm->scale[1], prm->scale[2]};
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    bvht_Triangle t = faces[i];
-    bvht_vec3 c = centroid(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+  // wave-uniform trips (the histogram adds use ballots)
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * 64; base < n; base += gridDim.x * kBlock) {
+    const uint32_t i = base + lane;
+    const bool valid = i < n;
+    const bvht_Triangle t = faces[valid ? i : base];
+    const bvht_vec3 c = centroid(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
     K code;
     if constexpr (sizeof(K) == 8)
       code = morton63(c, off, sc);
     else
       code = morton30(c, off, sc);
-    keys[i] = code;
+    if (valid) keys[i] = code;
 #pragma unroll
-    for (int p = 0; p < PASSES; ++p) atomicAdd(&s_hist[p][(uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1)], 1u);
+    for (int p = 0; p < PASSES; ++p)
+      waveHistAdd(s_hist[p], (uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1), valid);
   }
   __syncthreads();
 #pragma unroll
@@ -333,9 +338,12 @@ uint32_t boxBlocks(uint32_t n) {
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
+// Every Morton workgroup ends with one global atomic per non-empty bin of
+// every pass into the same 8 x 256 counters, so the grid is kept to two
+// workgroups per CU (each then covers ~8K triangles at C1).
 uint32_t mortonBlocks(uint32_t n) {
   uint32_t b = divUp(n, kBlock * 4);
-  uint32_t cap = (uint32_t)numCUs() * 8;
+  uint32_t cap = (uint32_t)numCUs() * 2;
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 

@@ -106,12 +106,13 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
   const bvht_Triangle *fc = faces + sd.faceBase;
   const uint32_t p0 = c.y << kChunkLog, p1 = min(p0 + (1u << kChunkLog), sd.n);
   constexpr int U = 4;
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += U * kBuildBlock) {
+  // trips are wave-uniform (the histogram adds use ballots)
+  for (uint32_t p = p0 + threadIdx.x; p - (threadIdx.x & 63) < p1; p += U * kBuildBlock) {
     bvht_Triangle tri[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t q = p + u * kBuildBlock;
-      tri[u] = fc[q < p1 ? q : p];
+      tri[u] = fc[q < p1 ? q : p0];  // p0 < p1: a chunk is never empty
     }
     bvht_vec3 va[U], vb[U], vc[U];
 #pragma unroll
@@ -123,11 +124,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t q = p + u * kBuildBlock;
-      if (q >= p1) continue;
+      const bool valid = q < p1;
       const uint32_t key = morton30(centroid(va[u], vb[u], vc[u]), off, sc);
-      keys[sd.faceBase + q] = key;
+      if (valid) keys[sd.faceBase + q] = key;
 #pragma unroll
-      for (int pass = 0; pass < P; ++pass) atomicAdd(&s_hist[pass * R + ((key >> (pass * RB)) & (R - 1))], 1u);
+      for (int pass = 0; pass < P; ++pass) waveHistAdd(s_hist + pass * R, (key >> (pass * RB)) & (R - 1), valid);
     }
   }
   __syncthreads();

@@ -28,6 +28,23 @@ __device__ __forceinline__ void fenceAcquireAgent() { __builtin_amdgcn_fence(__A
 
 __device__ __forceinline__ uint32_t divUpDev(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+// Adds one to bin `bin` of an LDS histogram for every valid lane of the
+// wave (every lane calls).  When all valid lanes hold the same bin (common
+// for the high digits of spatially coherent keys) one lane adds the count:
+// same-address LDS atomics from one instruction are serialised lane by lane.
+__device__ __forceinline__ void waveHistAdd(uint32_t *hist, uint32_t bin, bool valid) {
+  const uint64_t validMask = __ballot(valid);
+  if (validMask == 0) return;
+  const int first = __builtin_ctzll(validMask);
+  const uint32_t b0 = __builtin_amdgcn_readlane(bin, first);
+  const uint64_t differ = __ballot(valid && bin != b0);
+  if (differ == 0) {
+    if ((int)(threadIdx.x & 63) == first) atomicAdd(hist + b0, (uint32_t)__popcll(validMask));
+  } else if (valid) {
+    atomicAdd(hist + bin, 1u);
+  }
+}
+
 // Inclusive scan across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t x) {
   const uint32_t lane = threadIdx.x & 63;
