@@ -159,6 +159,7 @@ SIGNATURES = {
     "copyImageToHost": (None, [c_vp, c_vp, c_u32, c_u32]),
     "tracerDoCudaSync": (None, []),
     "bvht_traceRays": (c_bool, [c_vp, c_u32, c_vp, c_u32, MeshRaw, Vec3, ctypes.c_int, c_f, c_vp, c_vp, c_vp]),
+    "bvht_setRayOrdering": (None, [c_bool]),
     "bvht_makeBounceRays": (c_bool, [c_vp, c_vp, c_u32, MeshRaw, c_u32, c_vp, c_vp]),
     "bvht_sceneHeightfield": (None, [c_u32, c_u32, c_u32, c_f, c_vp, c_vp, P(c_u32), P(c_u32)]),
     "bvht_sceneSpheresSoup": (None, [c_u32, c_u32, c_u32, c_vp, c_vp, P(c_u32), P(c_u32)]),

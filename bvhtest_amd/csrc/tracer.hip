@@ -22,9 +22,11 @@
 //
 // Rays are generated in f64 from a camera basis computed in f64 on the host
 // (the exact inverse of lookAtRH), then rounded once to f32.
+#include "radix_sort.hpp"
 #include "runtime.hpp"
 #include "types.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace bvht {
@@ -415,10 +417,11 @@ __global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ r
                                                   uint32_t root, const bvht_vec3 *__restrict__ vert,
                                                   const bvht_Triangle *__restrict__ faces, bvht_vec3 light, float tMin,
                                                   const uint32_t *__restrict__ skip, bvht_Hit *__restrict__ hits,
-                                                  bvht_CUDAPixel *__restrict__ img) {
+                                                  bvht_CUDAPixel *__restrict__ img, const uint32_t *__restrict__ order) {
   __shared__ uint2 s_stk[kStackCap * 64];
-  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= num) return;
+  const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
+  if (slot >= num) return;
+  const uint32_t i = order ? order[slot] : slot;
   const bvht_Ray ray = rays[i];
   TraceOut o = {kNoHit, HUGE_VALF, 0};
   if (!(ray.dir.x == 0.0f && ray.dir.y == 0.0f && ray.dir.z == 0.0f))
@@ -499,6 +502,61 @@ __global__ __launch_bounds__(256) void k_bounce(const bvht_Ray *__restrict__ pri
   skip[i] = hh.triId;
 }
 
+__device__ __forceinline__ uint32_t spread9(uint32_t v) {  // 9 bits -> every third bit of 27
+  v &= 511u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t cell9(float x, float lo, float inv) {
+  const float q = (x - lo) * inv;
+  return q <= 0.0f ? 0u : (q >= 511.0f ? 511u : (uint32_t)q);
+}
+
+// Ordering key of a secondary ray: its direction cell (4 bins per axis over
+// [-1, 1]) above the Morton cell of its origin in the root box (8 bits per
+// axis).  Rays that start close together and head the same way walk similar
+// nodes, so a wave of them diverges less (measured at C1: bounce trace 5.6 ->
+// 4.9 ms with the sort included; octant-only direction cells 4.9, origin-major
+// keys 5.2).  A ray with no direction gets the largest key.  Plus the digit
+// histograms of the 4 sort passes.
+__global__ __launch_bounds__(256) void k_rayKeys(const bvht_Ray *__restrict__ rays, uint32_t num,
+                                                 const bvht_BVHNode *__restrict__ nodes, uint32_t root,
+                                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_hist[4][kSortRadix];
+  for (int p = 0; p < 4; ++p) s_hist[p][threadIdx.x] = 0;
+  __syncthreads();
+  const bvht_AABB box = nodes[root].bbox;
+  const float ix = 512.0f / fmaxf(box.minMax[1].x - box.minMax[0].x, 1e-30f);
+  const float iy = 512.0f / fmaxf(box.minMax[1].y - box.minMax[0].y, 1e-30f);
+  const float iz = 512.0f / fmaxf(box.minMax[1].z - box.minMax[0].z, 1e-30f);
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < num; i += gridDim.x * 256) {
+    const bvht_Ray r = rays[i];
+    uint32_t key = (1u << 30) - 1;
+    if (!(r.dir.x == 0.0f && r.dir.y == 0.0f && r.dir.z == 0.0f)) {
+      // direction: 4 bins per axis over [-1, 1]; origin: 8 bits per axis
+      const uint32_t dx = min((uint32_t)fmaxf((r.dir.x + 1.0f) * 2.0f, 0.0f), 3u);
+      const uint32_t dy = min((uint32_t)fmaxf((r.dir.y + 1.0f) * 2.0f, 0.0f), 3u);
+      const uint32_t dz = min((uint32_t)fmaxf((r.dir.z + 1.0f) * 2.0f, 0.0f), 3u);
+      const uint32_t dcell = (dx << 4) | (dy << 2) | dz;
+      const uint32_t o = (spread9(cell9(r.pos.x, box.minMax[0].x, ix) >> 1) << 2) |
+                         (spread9(cell9(r.pos.y, box.minMax[0].y, iy) >> 1) << 1) |
+                         spread9(cell9(r.pos.z, box.minMax[0].z, iz) >> 1);
+      key = (dcell << 24) | o;
+    }
+    keys[i] = key;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) atomicAdd(&s_hist[p][(key >> (8 * p)) & (kSortRadix - 1)], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < 4; ++p) {
+    const uint32_t c = s_hist[p][threadIdx.x];
+    if (c) atomicAdd(hist + p * kSortRadix + threadIdx.x, c);
+  }
+}
+
 // Packed copies of the tree and mesh, rebuilt by every trace call (the tree
 // may have changed since the last one); the buffers only grow.
 struct TraceCache {
@@ -507,6 +565,10 @@ struct TraceCache {
   PackedTri *tris = nullptr;
   size_t triCap = 0;
   uint32_t *bad = nullptr;
+  // ray ordering (bvht_traceRays): keys and indices x2, sort scratch
+  bool orderRays = true;
+  uint32_t orderCap = 0;
+  uint32_t *okeys[2] = {nullptr, nullptr}, *ovals[2] = {nullptr, nullptr}, *oscratch = nullptr;
 };
 TraceCache &traceCache() {
   static TraceCache c;
@@ -551,6 +613,35 @@ bool prepareTrace(const bvht_BVHNode *nodes, const bvht_MeshRaw &mesh, TraceCtx 
   return true;
 }
 
+// Sorts the rays by k_rayKeys and returns the trace order (nullptr on
+// failure, with the error recorded).
+const uint32_t *rayOrder(const bvht_Ray *rays, uint32_t num, const bvht_BVHNode *nodes, uint32_t root) {
+  TraceCache &c = traceCache();
+  const SortPlan plan = makeSortPlan(num, 4, 30);
+  if (c.orderCap < num) {
+    for (int b = 0; b < 2; ++b) {
+      devFree(c.okeys[b]);
+      devFree(c.ovals[b]);
+      c.okeys[b] = c.ovals[b] = nullptr;
+    }
+    devFree(c.oscratch);
+    c.oscratch = nullptr;
+    c.orderCap = 0;
+    if (!devAlloc(&c.okeys[0], num, "ray keys") || !devAlloc(&c.okeys[1], num, "ray keys") ||
+        !devAlloc(&c.ovals[0], num, "ray order") || !devAlloc(&c.ovals[1], num, "ray order") ||
+        !devMalloc(reinterpret_cast<void **>(&c.oscratch), sortScratchBytes(makeSortPlan(num, 4, 30)), "ray sort"))
+      return nullptr;
+    c.orderCap = num;
+  }
+  if (!sortReset(plan, c.oscratch)) return nullptr;
+  const uint32_t blocks = std::min<uint32_t>(divUp(num, 256 * 4), (uint32_t)numCUs() * 2);
+  k_rayKeys<<<blocks, 256, 0, stream()>>>(rays, num, nodes, root, c.okeys[0], sortHist(plan, c.oscratch));
+  if (!checkLaunch("k_rayKeys")) return nullptr;
+  void *keys[2] = {c.okeys[0], c.okeys[1]};
+  const int r = sortRun(plan, keys, c.ovals, true, false, c.oscratch);
+  return r < 0 ? nullptr : c.ovals[r];
+}
+
 }  // namespace
 
 }  // namespace bvht
@@ -558,6 +649,8 @@ bool prepareTrace(const bvht_BVHNode *nodes, const bvht_MeshRaw &mesh, TraceCtx 
 using namespace bvht;
 
 extern "C" {
+
+void bvht_setRayOrdering(bool enable) { traceCache().orderRays = enable; }
 
 bool allocateRays(bvht_Ray **rays, uint32_t num) { return devAlloc(rays, num, "allocateRays"); }
 bool allocateImage(bvht_CUDAPixel **img, uint32_t w, uint32_t h) {
@@ -642,12 +735,19 @@ bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uin
   if (num == 0) return true;
   TraceCtx cx;
   if (!prepareTrace(nodes, mesh, cx, "bvht_traceRays")) return false;
+  if (rootNode >= cx.bound) {
+    setError("bvht_traceRays: root %u outside the %u nodes of the mesh", rootNode, cx.bound);
+    return false;
+  }
+  // secondary rays only: primary rays of an image are coherent in their own order
+  const uint32_t *order = nullptr;  // (measured: ordering primary rays costs more than it saves)
+  if (mode == 1 && traceCache().orderRays && !(order = rayOrder(rays, num, nodes, rootNode))) return false;
   if (mode == 0)
     k_traceRays<0><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
-                                                       skipTri, hits, img);
+                                                       skipTri, hits, img, order);
   else
     k_traceRays<1><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
-                                                       skipTri, hits, img);
+                                                       skipTri, hits, img, order);
   return checkLaunch("k_traceRays");
 }
 

@@ -454,10 +454,14 @@ void tracerDoCudaSync(void);
 /* Extension: trace n rays with the same traversal and write per-ray hits
  * (and the reference pixel when img != NULL).  mode 0 = reference triangle
  * test (no t-min, CUDAKernels.cu:88-112); mode 1 = t > tMin and the triangle
- * `skipTri[i]` (if skipTri != NULL) is ignored, for secondary rays. */
+ * `skipTri[i]` (if skipTri != NULL) is ignored, for secondary rays.
+ * Unless switched off with bvht_setRayOrdering(false), mode-1 rays are traced
+ * in the order of a sort by (direction cell, origin cell); every ray's
+ * result is the same either way, only the memory locality differs. */
 bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
                     bvht_vec3 light, int mode, float tMin, const uint32_t *skipTri, bvht_Hit *hits,
                     bvht_CUDAPixel *img);
+void bvht_setRayOrdering(bool enable);
 /* Extension: one cosine-weighted diffuse bounce per primary hit: builds
  * secondary rays (origin hit + 1e-4 * n, n facing the incoming ray, direction
  * from a per-pixel hash of (seed, pixel)) into outRays; rays of missed pixels
