@@ -85,6 +85,21 @@ def profile_build(args):
     print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes}), flush=True)
 
 
+def profile_batch(args):
+    """Kernel-trace runs of the batched build: the 512-mesh C3 batch built
+    `steps` times without graph capture (tile 2^11, 8-bit digits), so every
+    kernel shows up under its own name."""
+    import bvhtest_amd as bt
+
+    L = bt.lib()
+    v, f, fc, vc = bt.scene_batch(512, 1000, 1000, 500000)
+    b = bt.BatchBuilder(v, f, fc, vc, tile_log2=11, radix_bits=8)
+    for _ in range(args.steps):
+        b.build(use_graph=False)
+    L.LBVH_doCUDASyc()
+    print(json.dumps({"profile_batch": True, "steps": args.steps, "triangles": int(fc.sum())}), flush=True)
+
+
 def measured_traffic(kernel, bits):
     """HBM bytes per launch of `kernel` from the committed counter runs
     (profiles/lbvh63_4M_traffic.json: FETCH_SIZE and WRITE_SIZE, calibrated
@@ -105,6 +120,8 @@ def run_bench(args):
         raise SystemExit("bench.py needs a HIP device")
     if args.profile_build:
         return profile_build(args)
+    if args.profile_batch:
+        return profile_batch(args)
     t0 = time.time()
     v, f = bt.scene_spheres_soup(2048, 500000, 2)
     n = len(f)
@@ -382,6 +399,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--profile-build", action="store_true", help="builds + a 256 MiB copy only (counter runs)")
+    ap.add_argument("--profile-batch", action="store_true", help="batched builds only (kernel-trace runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
