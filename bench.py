@@ -32,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 CAM = dict(pos=(0.5, 0.5, 2.0), look_at=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov=45.0)
 
 
-SORT_STAGES = ("k_msdScan", "k_msdScatter", "k_msdWindows", "k_msdBig", "k_onesweep")
+SORT_STAGES = ("k_msdScan", "k_msdScatter", "k_msdWindows", "k_msdSplit", "k_msdJobs", "k_onesweep", "k_splitRank",
+               "k_splitLong")
 
 
 def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
@@ -41,7 +42,9 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
     scene box: faces 12 n + vertices 12 nv; Morton: the same plus the key
     write; MSD scatter: key read, key + index written; onesweep pass: key +
     value read and written (first pass reads no value, last pass writes no
-    key); level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
+    key; 63-bit keys: the 5 passes over bits 24-62, all writing keys); group
+    rank: key 8 + index 4 read and written (each group member read once per
+    group, in cache); level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
     48-byte leaves (n) and the inner nodes below the top tiles
     (n - ceil(n / 2^tile_log2)) written.  The MSD window and big-bucket sorts
     move a data-dependent share of the pairs, so they have no byte model."""
@@ -53,9 +56,12 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
     if name == "k_msdScatter":
         return kb * n + (kb + 4) * n
     if name.startswith("k_onesweep"):
+        keep_last = morton_bits == 63  # 63-bit keys: the group step needs the sorted keys
         rd = kb + (0 if pass_idx == 0 else 4)
-        wr = (0 if pass_idx == passes - 1 else kb) + 4
+        wr = (0 if pass_idx == passes - 1 and not keep_last else kb) + 4
         return n * (rd + wr)
+    if name == "k_splitRank":
+        return 2 * (8 * n + 4 * n)
     if name == "k_buildLevel0":
         inner = n - -(-n // (1 << tile_log2))
         return 16 * n + 12 * nv + 48 * (n + inner)

@@ -19,6 +19,7 @@
 #include "lbvh_kernels.hpp"
 #include "msd_sort.hpp"
 #include "radix_sort.hpp"
+#include "split_sort.hpp"
 #include "refit.hpp"
 #include "wave.hpp"
 
@@ -283,9 +284,10 @@ __global__ __launch_bounds__(kBlock) void k_leaves(bvht_BVHNode *__restrict__ no
 }
 
 struct Internal {
-  SortPlan plan;  // onesweep (digitBits 8, the default)
-  MsdPlan msd;    // MSD sort (digitBits 16)
-  bool useMsd = false;
+  SortPlan plan;    // onesweep (digitBits 8, the default; 30-bit keys)
+  SplitPlan split;  // digitBits 8 with 63-bit keys: top 30 bits, then the groups
+  MsdPlan msd;      // MSD sort (digitBits 16)
+  bool useMsd = false, useSplit = false;
   float *partials = nullptr;
   uint32_t boxBlocks = 0;
   int resultBuf = 0;
@@ -348,7 +350,11 @@ uint32_t mortonBlocks(uint32_t n) {
 }
 
 // Words of the sort scratch that must be zero before the key producer runs.
-size_t sortZeroWords(const Internal *in) { return in->useMsd ? in->msd.zeroWords : in->plan.totalWords; }
+size_t sortZeroWords(const Internal *in) {
+  if (in->useMsd) return in->msd.zeroWords;
+  if (in->useSplit) return in->split.ctrlOff + 16;
+  return in->plan.totalWords;
+}
 
 bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
   Internal *in = internalOf(mem);
@@ -366,7 +372,8 @@ bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
                                                               (uint32_t *)mem->keys[0], hist, shift);
     return checkLaunch("k_mortonMsd");
   }
-  uint32_t *hist = sortHist(in->plan, (uint32_t *)mem->sortScratch);
+  uint32_t *hist = in->useSplit ? splitHist(in->split, (uint32_t *)mem->sortScratch)
+                                : sortHist(in->plan, (uint32_t *)mem->sortScratch);
   if (mem->opts.mortonBits == 63)
     k_morton<uint64_t, 8><<<mortonBlocks(n), kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
                                                                     (uint64_t *)mem->keys[0], hist);
@@ -388,6 +395,17 @@ bool runSort(bvht_LBVH_WorkingMemory *mem) {
     in->resultBuf = 1;
     mem->sortedValues = mem->values[1];
     mem->sortedKeys = keepKeys ? mem->keys[0] : nullptr;
+    return true;
+  }
+  if (in->useSplit) {
+    uint64_t *k2[2] = {(uint64_t *)mem->keys[0], (uint64_t *)mem->keys[1]};
+    const int r = splitRun(in->split, k2, mem->values, (uint32_t *)mem->sortScratch,
+                           tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr,
+                           tm.enabled ? [](const char *s) { timing().record(s); } : nullptr);
+    if (r < 0) return false;
+    in->resultBuf = r;
+    mem->sortedValues = mem->values[r];
+    mem->sortedKeys = keepKeys ? mem->keys[r] : nullptr;
     return true;
   }
   int r = sortRun(in->plan, mem->keys, mem->values, true, keepKeys, (uint32_t *)mem->sortScratch,
@@ -565,9 +583,13 @@ bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *me
   m.internal = in;
   m.numFaces = n;
   in->useMsd = m.opts.digitBits == kMsdTopBits;
+  in->useSplit = !in->useMsd && m.opts.mortonBits == 63;
   if (in->useMsd) {
     in->msd = makeMsdPlan(n, keyBytes, (int)m.opts.mortonBits);
     m.sortScratchBytes = msdScratchBytes(in->msd);
+  } else if (in->useSplit) {
+    in->split = makeSplitPlan(n);
+    m.sortScratchBytes = splitScratchBytes(in->split);
   } else {
     in->plan = makeSortPlan(n, keyBytes, (int)m.opts.mortonBits);
     m.sortScratchBytes = sortScratchBytes(in->plan);
@@ -735,13 +757,15 @@ uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem) {
     return 0xFFFFFFFFu;
   }
   const Internal *in = internalOf(mem);
-  const size_t word = in->useMsd ? in->msd.ctrlOff + 2 : in->plan.counterOff + 8;
-  uint32_t v = 0xFFFFFFFFu;
-  if (!check(hipMemcpyAsync(&v, (uint32_t *)mem->sortScratch + word, 4, hipMemcpyDeviceToHost, stream()),
-             "bvht_LBVH_sortStatus") ||
+  const uint32_t *sc = (const uint32_t *)mem->sortScratch;
+  const size_t word = in->useMsd ? in->msd.ctrlOff + 2 : (in->useSplit ? in->split.plan.counterOff : in->plan.counterOff) + 8;
+  uint32_t v = 0xFFFFFFFFu, w = 0;
+  if (!check(hipMemcpyAsync(&v, sc + word, 4, hipMemcpyDeviceToHost, stream()), "bvht_LBVH_sortStatus") ||
+      (in->useSplit &&
+       !check(hipMemcpyAsync(&w, sc + in->split.ctrlOff + 2, 4, hipMemcpyDeviceToHost, stream()), "bvht_LBVH_sortStatus")) ||
       !check(hipStreamSynchronize(stream()), "bvht_LBVH_sortStatus sync"))
     return 0xFFFFFFFFu;
-  return v;
+  return v | (w << 3);
 }
 
 void bvht_LBVH_setTiming(bool enable) { timing().enabled = enable; }

@@ -1,6 +1,7 @@
 // Most-significant-digit sort of (key, input index) pairs (see msd_sort.hpp).
 #include "msd_sort.hpp"
 #include "radix_sort.hpp"
+#include "split_sort.hpp"
 
 #include <algorithm>
 
@@ -655,6 +656,25 @@ bool bvht_sortKeyIndexPairs(const void *keys, uint32_t n, uint32_t keyBits, uint
     }
     ok = ok && check(hipMemcpyAsync(valsOut, v1, 4 * (size_t)n, hipMemcpyDeviceToDevice, stream()), "sort value copy") &&
          (!keysOut || check(hipMemcpyAsync(keysOut, k0, kb, hipMemcpyDeviceToDevice, stream()), "sort key copy"));
+  } else if (ok && keyBits == 63) {
+    const SplitPlan p = makeSplitPlan(n);
+    uint64_t *kk[2] = {(uint64_t *)k0, (uint64_t *)k1};
+    uint32_t *vv[2] = {v0, v1};
+    ok = devMalloc((void **)&scratch, splitScratchBytes(p), "sort scratch") && splitReset(p, scratch) &&
+         sortHistogram(p.plan, k0, scratch);
+    const int r = ok ? splitRun(p, kk, vv, scratch) : -1;
+    uint32_t words[2] = {0, 0};
+    ok = r >= 0 &&
+         check(hipMemcpyAsync(&words[0], scratch + p.plan.counterOff + 8, 4, hipMemcpyDeviceToHost, stream()),
+               "sort status") &&
+         check(hipMemcpyAsync(&words[1], scratch + p.ctrlOff + 2, 4, hipMemcpyDeviceToHost, stream()), "sort status") &&
+         check(hipStreamSynchronize(stream()), "sort sync");
+    if (ok && (words[0] | words[1])) {
+      setError("bvht_sortKeyIndexPairs: sort error bits 0x%x", words[0] | (words[1] << 3));
+      ok = false;
+    }
+    ok = ok && check(hipMemcpyAsync(valsOut, vv[r], 4 * (size_t)n, hipMemcpyDeviceToDevice, stream()), "sort value copy") &&
+         (!keysOut || check(hipMemcpyAsync(keysOut, kk[r], kb, hipMemcpyDeviceToDevice, stream()), "sort key copy"));
   } else if (ok) {
     const SortPlan p = makeSortPlan(n, keyBytes, (int)keyBits);
     void *kk[2] = {k0, k1};

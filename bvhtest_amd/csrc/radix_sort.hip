@@ -203,14 +203,14 @@ bool sortHistogram(const SortPlan &p, const void *keys, uint32_t *scratch) {
 }
 
 int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityValues, bool keepKeys, uint32_t *scratch,
-            void (*afterPass)(int)) {
+            void (*afterPass)(int), int firstPass) {
   if (p.n == 0) return 0;
   int cur = 0;
-  for (int pass = 0; pass < p.passes; ++pass) {
+  for (int pass = firstPass; pass < p.passes; ++pass) {
     const uint32_t *hist = scratch + p.histOff + (size_t)pass * kSortRadix;
     uint32_t *counter = scratch + p.counterOff + pass;
     uint32_t *status = scratch + p.statusOff + (size_t)pass * p.numTiles * kSortRadix;
-    const uint32_t *vin = (pass == 0 && identityValues) ? nullptr : vals[cur];
+    const uint32_t *vin = (pass == firstPass && identityValues) ? nullptr : vals[cur];
     bool last = pass == p.passes - 1;
     void *kout = (last && !keepKeys) ? nullptr : keys[cur ^ 1];
     uint32_t shift = pass * kSortRadixBits;

@@ -1,0 +1,228 @@
+// Two-step sort of (63-bit key, input index) pairs (see split_sort.hpp).
+#include "split_sort.hpp"
+#include "wave.hpp"
+
+
+namespace bvht {
+
+namespace {
+
+enum : uint32_t { kErrList = 1 };
+constexpr int kLongItems = 16;  // pairs per thread per chunk of the long-group sort
+constexpr uint32_t kLongChunk = kSortBlock * kLongItems;
+
+__device__ __forceinline__ uint64_t hiOf(uint64_t k) { return k >> kSplitLoBits; }
+
+// First and one-past-last position of the run of top bits `hi` around p
+// (the array is sorted by top bits): galloping out from p, then a binary
+// search, so a pair of a group of m pairs costs O(log m) loads.
+__device__ __forceinline__ uint32_t groupStart(const uint64_t *kf, uint32_t p, uint64_t hi) {
+  uint32_t q = p, step = 1;
+  while (q >= step && hiOf(kf[q - step]) == hi) {  // kf[q] is in the group
+    q -= step;
+    step <<= 1;
+  }
+  uint32_t lo = q >= step ? q - step + 1 : 0, h = q;  // the start lies in [lo, h]
+  while (lo < h) {
+    const uint32_t mid = lo + (h - lo) / 2;
+    if (hiOf(kf[mid]) == hi)
+      h = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t groupEnd(const uint64_t *kf, uint32_t n, uint32_t p, uint64_t hi) {
+  uint32_t q = p, step = 1;
+  while (q + step < n && hiOf(kf[q + step]) == hi) {
+    q += step;
+    step <<= 1;
+  }
+  uint32_t lo = q, h = q + step < n ? q + step : n;  // the last member lies in [lo, h)
+  while (h - lo > 1) {
+    const uint32_t mid = lo + (h - lo) / 2;
+    if (hiOf(kf[mid]) == hi)
+      lo = mid;
+    else
+      h = mid;
+  }
+  return lo + 1;
+}
+
+// Each pair of a group of equal top bits (m <= kSplitGroupMax) goes to
+// group start + its rank by (key, index) inside the group (O(m) compares;
+// the lanes of a wave inside one group read the same member at once).  A
+// longer group is listed once, by its first pair, for k_splitLong.
+__global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ kf, const uint32_t *__restrict__ vals,
+                                                   uint32_t n, uint32_t *__restrict__ valsOut,
+                                                   uint64_t *__restrict__ keysOut, uint32_t *__restrict__ ctrl,
+                                                   uint2 *__restrict__ list, uint32_t listCap) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t key = kf[p], hi = hiOf(key);
+  const uint32_t v = vals[p];
+  const uint32_t gs = groupStart(kf, p, hi), ge = groupEnd(kf, n, p, hi);
+  if (ge - gs > kSplitGroupMax) {
+    if (p == gs) {
+      const uint32_t slot = atomicAdd(ctrl + 0, 1u);
+      if (slot < listCap)
+        list[slot] = make_uint2(gs, ge - gs);
+      else
+        atomicOr(ctrl + 2, kErrList);
+    }
+    return;
+  }
+  uint32_t rank = 0;
+#pragma unroll 4
+  for (uint32_t q = gs; q < ge; ++q) {
+    const uint64_t kq = kf[q];
+    const uint32_t vq = vals[q];
+    rank += (kq < key || (kq == key && vq < v)) ? 1u : 0u;
+  }
+  valsOut[gs + rank] = v;
+  if (keysOut) keysOut[gs + rank] = key;
+}
+
+// Long groups: one workgroup per group, a stable LSD over the low 24 bits
+// (3 passes of 8 bits) through global memory.  The group is in input order
+// on entry (the top-bit sort is stable), so the result is ordered by
+// (key, index).  Pass 1 reads (kA, vA) and writes (kB, vB), pass 2 the
+// reverse, pass 3 again (kA, vA) -> (kB, vB).  Chunks of
+// kLongChunk pairs are ranked stably with wave ballots (item-major order in
+// a wave, waves in order) and placed after the digit's running offset.
+__global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *__restrict__ kA, uint32_t *__restrict__ vA,
+                                                          uint64_t *__restrict__ kB, uint32_t *__restrict__ vB,
+                                                          const uint2 *__restrict__ list, uint32_t *__restrict__ ctrl,
+                                                          uint32_t listCap) {
+  constexpr int WAVES = kSortBlock / 64;
+  __shared__ uint32_t s_hist[kSortRadix], s_off[kSortRadix], s_wave[WAVES][kSortRadix], s_scan[WAVES];
+  __shared__ uint32_t s_item;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t num = min(ctrl[0], listCap);
+  const uint64_t lt = __lanemask_lt();
+  while (true) {
+    if (tid == 0) s_item = atomicAdd(ctrl + 1, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    __syncthreads();  // s_item is rewritten only after this barrier
+    if (item >= num) break;
+    const uint2 g = list[item];
+    const uint32_t s = g.x, m = g.y;
+    for (int pass = 0; pass < kSplitFirstPass; ++pass) {
+      const uint32_t shift = 8 * pass;
+      const uint64_t *ki = (pass & 1) ? kB : kA;
+      const uint32_t *vi = (pass & 1) ? vB : vA;
+      uint64_t *ko = (pass & 1) ? kA : kB;
+      uint32_t *vo = (pass & 1) ? vA : vB;
+      s_hist[tid] = 0;
+      __syncthreads();
+      for (uint32_t c0 = 0; c0 < m; c0 += kLongChunk) {
+        uint32_t d[kLongItems];
+#pragma unroll
+        for (int i = 0; i < kLongItems; ++i) {
+          const uint32_t j = c0 + tid + i * kSortBlock;
+          d[i] = j < m ? (uint32_t)(ki[s + j] >> shift) & (kSortRadix - 1) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int i = 0; i < kLongItems; ++i)
+          if (d[i] != 0xFFFFFFFFu) atomicAdd(&s_hist[d[i]], 1u);
+      }
+      __syncthreads();
+      const uint32_t c = s_hist[tid];
+      s_off[tid] = blockExclusiveScan<kSortBlock>(c, s_scan);
+      __syncthreads();
+      for (uint32_t c0 = 0; c0 < m; c0 += kLongChunk) {
+        const uint32_t base = c0 + wave * 64 * kLongItems;
+        uint64_t key[kLongItems];
+        uint32_t val[kLongItems], rank[kLongItems];
+#pragma unroll
+        for (int i = 0; i < kLongItems; ++i) {
+          const uint32_t j = base + i * 64 + lane;
+          key[i] = j < m ? ki[s + j] : 0ull;
+          val[i] = j < m ? vi[s + j] : 0u;
+        }
+        for (int w = 0; w < WAVES; ++w) s_wave[w][tid] = 0;
+        __syncthreads();
+        uint32_t *cnt = &s_wave[wave][0];
+#pragma unroll
+        for (int i = 0; i < kLongItems; ++i) {
+          const bool valid = base + i * 64 + lane < m;
+          const uint32_t dg = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
+          uint64_t peers = __ballot(valid);
+#pragma unroll
+          for (int b = 0; b < kSortRadixBits; ++b) {
+            const uint64_t mb = __ballot((dg >> b) & 1u);
+            peers &= ((dg >> b) & 1u) ? mb : ~mb;
+          }
+          const uint32_t lower = (uint32_t)__popcll(peers & lt);
+          const uint32_t cc = __hip_atomic_load(cnt + dg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          rank[i] = cc + lower;
+          if (valid && lower == 0)
+            __hip_atomic_store(cnt + dg, cc + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+        // per digit: exclusive offsets of the waves inside the chunk, chunk total
+        uint32_t run = 0;
+        for (int w = 0; w < WAVES; ++w) {
+          const uint32_t x = s_wave[w][tid];
+          s_wave[w][tid] = run;
+          run += x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kLongItems; ++i) {
+          if (base + i * 64 + lane < m) {
+            const uint32_t dg = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
+            const uint32_t pos = s_off[dg] + s_wave[wave][dg] + rank[i];
+            ko[s + pos] = key[i];
+            vo[s + pos] = val[i];
+          }
+        }
+        __syncthreads();
+        s_off[tid] += run;  // the chunk's pairs of digit tid now precede the next chunk's
+        __syncthreads();
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+SplitPlan makeSplitPlan(uint32_t n) {
+  SplitPlan p;
+  p.n = n;
+  p.plan = makeSortPlan(n, 8, 63);
+  p.listCap = n / (kSplitGroupMax + 1) + 1;
+  p.ctrlOff = p.plan.totalWords;
+  p.listOff = p.ctrlOff + 16;
+  p.totalWords = p.listOff + 2 * (size_t)p.listCap;
+  return p;
+}
+
+bool splitReset(const SplitPlan &p, uint32_t *scratch) {
+  return sortReset(p.plan, scratch) && fillU32(scratch + p.ctrlOff, 0u, 16);
+}
+
+int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch, void (*afterPass)(int),
+             void (*afterStage)(const char *)) {
+  if (p.n == 0) return 0;
+  void *kv[2] = {keys[0], keys[1]};
+  const int r = sortRun(p.plan, kv, vals, true, true, scratch, afterPass, kSplitFirstPass);
+  if (r < 0) return -1;
+  uint32_t *ctrl = scratch + p.ctrlOff;
+  uint2 *list = reinterpret_cast<uint2 *>(scratch + p.listOff);
+  k_splitRank<<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], vals[r], p.n, vals[r ^ 1], keys[r ^ 1], ctrl, list,
+                                                     p.listCap);
+  if (!checkLaunch("k_splitRank")) return -1;
+  if (afterStage) afterStage("k_splitRank");
+  // long groups are rare (none at the 4.24M-triangle scene): a small grid
+  // keeps the launch cheap when the list is empty
+  k_splitLong<<<32, kSortBlock, 0, stream()>>>(keys[r], vals[r], keys[r ^ 1], vals[r ^ 1], list, ctrl, p.listCap);
+  if (!checkLaunch("k_splitLong")) return -1;
+  if (afterStage) afterStage("k_splitLong");
+  return r ^ 1;
+}
+
+}  // namespace bvht

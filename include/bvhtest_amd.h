@@ -189,9 +189,11 @@ typedef struct bvht_LBVH_Options {
                             the full 64-bit key (morton<<32 | index) */
   uint32_t tileLog2;     /* log2 of leaves per build workgroup (8..11), 0 = default */
   uint32_t digitBits;    /* first sort digit: 8 (default, 0 = default) = onesweep LSD radix sort,
-                            8-bit digits; 16 = most-significant-digit split on the top 16 key
-                            bits, then in-LDS sorts of the buckets by (key, index).  Both give
-                            the permutation of the reference's stable sort. */
+                            8-bit digits (63-bit keys: over the top 30 bits, then each group of
+                            equal top bits ordered by (key, index)); 16 = most-significant-digit
+                            split on the top 16 key bits, then in-LDS sorts of the buckets by
+                            (key, index).  All give the permutation of the reference's stable
+                            sort. */
 } bvht_LBVH_Options;
 
 /* LBVH_WorkingMemory -- lib/builder/LBVH_CUDA.hpp:37-47 (scratch handles). */
@@ -243,7 +245,8 @@ bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvh
 /* Error bits of the last sort (extension, synchronises): 0 = the sort ran
  * to completion.  MSD sort: 1 = histogram does not match the keys, 2 = a
  * bucket ran out of key bits, 4 = a piece exceeded the in-LDS limit;
- * onesweep: 1 = a look-back wait gave up.
+ * onesweep: 1 = a look-back wait gave up; 63-bit keys also 8 = the list of
+ * long groups of equal top bits overflowed.
  * 0xFFFFFFFF on a bad argument or a failed copy. */
 uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem);
 /* Sorts the n (key, index) pairs of device keys[0..n) (keyBits 30: uint32
