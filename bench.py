@@ -93,13 +93,14 @@ def profile_build(args):
 
 def profile_batch(args):
     """Kernel-trace runs of the batched build: the 512-mesh C3 batch built
-    `steps` times without graph capture (tile 2^11, 8-bit digits), so every
-    kernel shows up under its own name."""
+    `steps` times without graph capture (tile 2^10 and 8-bit digits, the
+    autotuner's choice on MI355X), so every kernel shows up under its own
+    name."""
     import bvhtest_amd as bt
 
     L = bt.lib()
     v, f, fc, vc = bt.scene_batch(512, 1000, 1000, 500000)
-    b = bt.BatchBuilder(v, f, fc, vc, tile_log2=11, radix_bits=8)
+    b = bt.BatchBuilder(v, f, fc, vc, tile_log2=10, radix_bits=8)
     for _ in range(args.steps):
         b.build(use_graph=False)
     L.LBVH_doCUDASyc()
