@@ -32,48 +32,35 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTileLog = 10;  // default: 1024 units per build workgroup
+// Scene box: 1024-thread workgroups (16 waves hide the face -> vertex load
+// chain), two per CU.
+constexpr int kBoxBlock = 1024;
 constexpr uint32_t kBoxBlocksMax = 1024;
 
 __device__ __forceinline__ bvht_vec3 ldVec3(const bvht_vec3 *p) { return *p; }
 
-__global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict__ vert,
-                                                     const bvht_Triangle *__restrict__ faces, uint32_t n,
-                                                     BuildParams *__restrict__ prm, float *__restrict__ partials,
-                                                     uint32_t *__restrict__ zeroPtr, size_t zeroWords) {
+__global__ __launch_bounds__(kBoxBlock) void k_sceneBox(const bvht_vec3 *__restrict__ vert,
+                                                        const bvht_Triangle *__restrict__ faces, uint32_t n,
+                                                        BuildParams *__restrict__ prm, float *__restrict__ partials,
+                                                        uint32_t *__restrict__ zeroPtr, size_t zeroWords) {
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (size_t i = (size_t)blockIdx.x * kBlock + tid; i < zeroWords; i += (size_t)gridDim.x * kBlock) zeroPtr[i] = 0;
+  for (size_t i = (size_t)blockIdx.x * kBoxBlock + tid; i < zeroWords; i += (size_t)gridDim.x * kBoxBlock) zeroPtr[i] = 0;
   // the reduction starts from the all-zero box (LBVH_CUDA.cu:104-106)
   float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
-  // four triangles per trip, every load issued before the first is used
-  constexpr int R = 4;
-  const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t i0 = blockIdx.x * kBlock + tid; i0 < n; i0 += R * stride) {
-    bvht_Triangle t[R];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      const uint32_t i = i0 + u * stride;
-      t[u] = i < n ? faces[i] : faces[i0];
-    }
-    bvht_vec3 a[R], b[R], c[R];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      a[u] = ldVec3(vert + t[u].v1);
-      b[u] = ldVec3(vert + t[u].v2);
-      c[u] = ldVec3(vert + t[u].v3);
-    }
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      // a repeat of triangle i0 for a slot past the end changes no min/max
-      const bvht_AABB bx = triangleBox(a[u], b[u], c[u]);
-      mn[0] = mergeMin(mn[0], bx.minMax[0].x);
-      mn[1] = mergeMin(mn[1], bx.minMax[0].y);
-      mn[2] = mergeMin(mn[2], bx.minMax[0].z);
-      mx[0] = mergeMax(mx[0], bx.minMax[1].x);
-      mx[1] = mergeMax(mx[1], bx.minMax[1].y);
-      mx[2] = mergeMax(mx[2], bx.minMax[1].z);
-    }
+  // a wave takes 64 consecutive triangles per trip (coalesced face loads)
+  for (uint32_t base = (blockIdx.x * (kBoxBlock / 64) + wave) * 64; base < n; base += gridDim.x * kBoxBlock) {
+    const uint32_t i = base + lane;
+    // a repeat of triangle `base` for a lane past the end changes no min/max
+    const bvht_Triangle t = faces[i < n ? i : base];
+    const bvht_AABB bx = triangleBox(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+    mn[0] = mergeMin(mn[0], bx.minMax[0].x);
+    mn[1] = mergeMin(mn[1], bx.minMax[0].y);
+    mn[2] = mergeMin(mn[2], bx.minMax[0].z);
+    mx[0] = mergeMax(mx[0], bx.minMax[1].x);
+    mx[1] = mergeMax(mx[1], bx.minMax[1].y);
+    mx[2] = mergeMax(mx[2], bx.minMax[1].z);
   }
-  __shared__ float s_red[6][kBlock / 64];
+  __shared__ float s_red[6][kBoxBlock / 64];
   __shared__ bool s_last;
   for (int a = 0; a < 3; ++a) {
     float x = waveMin(mn[a]), y = waveMax(mx[a]);
@@ -86,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict
   if (tid == 0) {
     for (int a = 0; a < 3; ++a) {
       float x = s_red[a][0], y = s_red[3 + a][0];
-      for (int w = 1; w < kBlock / 64; ++w) {
+      for (int w = 1; w < kBoxBlock / 64; ++w) {
         x = mergeMin(x, s_red[a][w]);
         y = mergeMax(y, s_red[3 + a][w]);
       }
@@ -103,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict
     mn[a] = 0.0f;
     mx[a] = 0.0f;
   }
-  for (uint32_t b = tid; b < gridDim.x; b += kBlock)
+  for (uint32_t b = tid; b < gridDim.x; b += kBoxBlock)
     for (int a = 0; a < 3; ++a) {
       mn[a] = mergeMin(mn[a], partials[b * 6 + a]);
       mx[a] = mergeMax(mx[a], partials[b * 6 + 3 + a]);
@@ -120,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict
   if (tid == 0) {
     for (int a = 0; a < 3; ++a) {
       float x = s_red[a][0], y = s_red[3 + a][0];
-      for (int w = 1; w < kBlock / 64; ++w) {
+      for (int w = 1; w < kBoxBlock / 64; ++w) {
         x = mergeMin(x, s_red[a][w]);
         y = mergeMax(y, s_red[3 + a][w]);
       }
@@ -135,19 +122,24 @@ __global__ __launch_bounds__(kBlock) void k_sceneBox(const bvht_vec3 *__restrict
   }
 }
 
-template <typename K, int PASSES>
-__global__ __launch_bounds__(kBlock) void k_morton(const bvht_vec3 *__restrict__ vert,
-                                                   const bvht_Triangle *__restrict__ faces, uint32_t n,
-                                                   const BuildParams *__restrict__ prm, K *__restrict__ keys,
-                                                   uint32_t *__restrict__ hist) {
+// Morton keys plus the histograms of the digit passes FIRST..PASSES-1 (the
+// passes the sort runs).  Workgroups of kMortonBlock threads: 16 waves per
+// workgroup hide the face -> vertex load chain while the grid stays at two
+// workgroups per CU (one global flush of the histograms per workgroup).
+constexpr int kMortonBlock = 1024;
+template <typename K, int PASSES, int FIRST>
+__global__ __launch_bounds__(kMortonBlock) void k_morton(const bvht_vec3 *__restrict__ vert,
+                                                         const bvht_Triangle *__restrict__ faces, uint32_t n,
+                                                         const BuildParams *__restrict__ prm, K *__restrict__ keys,
+                                                         uint32_t *__restrict__ hist) {
   __shared__ uint32_t s_hist[PASSES][kSortRadix];
-  for (int p = 0; p < PASSES; ++p) s_hist[p][threadIdx.x] = 0;
+  for (uint32_t j = threadIdx.x; j < PASSES * kSortRadix; j += kMortonBlock) (&s_hist[0][0])[j] = 0;
   __syncthreads();
   const bvht_vec3 off = {prm->offset[0], prm->offset[1], prm->offset[2]};
   const bvht_vec3 sc = {prm->scale[0], prm->scale[1], prm->scale[2]};
   // wave-uniform trips (the histogram adds use ballots)
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * 64; base < n; base += gridDim.x * kBlock) {
+  for (uint32_t base = (blockIdx.x * (kMortonBlock / 64) + wave) * 64; base < n; base += gridDim.x * kMortonBlock) {
     const uint32_t i = base + lane;
     const bool valid = i < n;
     const bvht_Triangle t = faces[valid ? i : base];
@@ -159,14 +151,13 @@ __global__ __launch_bounds__(kBlock) void k_morton(const bvht_vec3 *__restrict__
       code = morton30(c, off, sc);
     if (valid) keys[i] = code;
 #pragma unroll
-    for (int p = 0; p < PASSES; ++p)
+    for (int p = FIRST; p < PASSES; ++p)
       waveHistAdd(s_hist[p], (uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1), valid);
   }
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p) {
-    uint32_t c = s_hist[p][threadIdx.x];
-    if (c) atomicAdd(&hist[p * kSortRadix + threadIdx.x], c);
+  for (uint32_t j = FIRST * kSortRadix + threadIdx.x; j < PASSES * kSortRadix; j += kMortonBlock) {
+    const uint32_t c = (&s_hist[0][0])[j];
+    if (c) atomicAdd(&hist[j], c);
   }
 }
 
@@ -335,16 +326,16 @@ bool validateMesh(const bvht_MeshRaw *mesh, const char *what) {
 }
 
 uint32_t boxBlocks(uint32_t n) {
-  uint32_t b = divUp(n, kBlock * 8);
-  uint32_t cap = std::min<uint32_t>(kBoxBlocksMax, (uint32_t)numCUs() * 4);
+  uint32_t b = divUp(n, kBoxBlock * 4);
+  uint32_t cap = std::min<uint32_t>(kBoxBlocksMax, (uint32_t)numCUs() * 2);
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
 // Every Morton workgroup ends with one global atomic per non-empty bin of
 // every pass into the same 8 x 256 counters, so the grid is kept to two
-// workgroups per CU (each then covers ~8K triangles at C1).
+// workgroups per CU (each then covers ~8K triangles at 4.24M triangles).
 uint32_t mortonBlocks(uint32_t n) {
-  uint32_t b = divUp(n, kBlock * 4);
+  uint32_t b = divUp(n, kMortonBlock * 4);
   uint32_t cap = (uint32_t)numCUs() * 2;
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
@@ -375,11 +366,11 @@ bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
   uint32_t *hist = in->useSplit ? splitHist(in->split, (uint32_t *)mem->sortScratch)
                                 : sortHist(in->plan, (uint32_t *)mem->sortScratch);
   if (mem->opts.mortonBits == 63)
-    k_morton<uint64_t, 8><<<mortonBlocks(n), kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
-                                                                    (uint64_t *)mem->keys[0], hist);
+    k_morton<uint64_t, 8, kSplitFirstPass><<<mortonBlocks(n), kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, (const BuildParams *)mem->params, (uint64_t *)mem->keys[0], hist);
   else
-    k_morton<uint32_t, 4><<<mortonBlocks(n), kBlock, 0, stream()>>>(mesh->vert, mesh->faces, n, (const BuildParams *)mem->params,
-                                                                    (uint32_t *)mem->keys[0], hist);
+    k_morton<uint32_t, 4, 0><<<mortonBlocks(n), kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, (const BuildParams *)mem->params, (uint32_t *)mem->keys[0], hist);
   return checkLaunch("k_morton");
 }
 
@@ -656,7 +647,7 @@ bvht_AABB LBVH_initTriData(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mes
   Internal *in = internalOf(mem);
   in->mesh = *mesh;
   in->haveMesh = true;
-  k_sceneBox<<<in->boxBlocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
+  k_sceneBox<<<in->boxBlocks, kBoxBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
                                                      in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
   if (!checkLaunch("k_sceneBox")) return box;
   BuildParams p;
@@ -740,7 +731,7 @@ bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvh
   Timing &tm = timing();
   tm.start();
   tm.record("begin");
-  k_sceneBox<<<in->boxBlocks, kBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
+  k_sceneBox<<<in->boxBlocks, kBoxBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
                                                      in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
   if (!checkLaunch("k_sceneBox")) return false;
   tm.record("k_sceneBox");

@@ -205,12 +205,7 @@ __global__ __launch_bounds__(kSortBlock) void k_segOnesweep(const uint32_t *__re
   for (int i = 0; i < kSegItems; ++i) {
     const bool valid = base + i * 64 + lane < segN;
     const uint32_t d = (key[i] >> shift) & (R - 1);
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < RB; ++b) {
-      const uint64_t m = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? m : ~m;
-    }
+    const uint64_t peers = matchDigit<RB>(d, __ballot(valid));
     const uint32_t lower = (uint32_t)__popcll(peers & lt64);
     const uint32_t c = __hip_atomic_load(cnt + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     rank[i] = c + lower;

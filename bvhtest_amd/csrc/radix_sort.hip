@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
   }
 
   // Stable in-wave ranking: item-major order inside a wave's segment equals
-  // tile order.  Peers (same digit) are found with 8 ballots; the lowest
+  // tile order.  Peers (same digit) are found with 8 ballots (matchDigit); the lowest
   // peer bumps the wave's counter for the digit.  The counters are read and
   // written with relaxed workgroup-scope atomics: plain LDS instructions the
   // compiler may not cache or reorder (a volatile pointer would become a
@@ -80,12 +80,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
   for (int i = 0; i < ITEMS; ++i) {
     bool valid = base + i * 64 + lane < n;
     uint32_t d = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < kSortRadixBits; ++b) {
-      uint64_t m = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? m : ~m;
-    }
+    const uint64_t peers = matchDigit<kSortRadixBits>(d, __ballot(valid));
     uint32_t lower = (uint32_t)__popcll(peers & lt);
     uint32_t c = __hip_atomic_load(cnt + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     rank[i] = c + lower;

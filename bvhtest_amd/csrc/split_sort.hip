@@ -148,12 +148,7 @@ __global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *__restrict__
         for (int i = 0; i < kLongItems; ++i) {
           const bool valid = base + i * 64 + lane < m;
           const uint32_t dg = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
-          uint64_t peers = __ballot(valid);
-#pragma unroll
-          for (int b = 0; b < kSortRadixBits; ++b) {
-            const uint64_t mb = __ballot((dg >> b) & 1u);
-            peers &= ((dg >> b) & 1u) ? mb : ~mb;
-          }
+          const uint64_t peers = matchDigit<kSortRadixBits>(dg, __ballot(valid));
           const uint32_t lower = (uint32_t)__popcll(peers & lt);
           const uint32_t cc = __hip_atomic_load(cnt + dg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           rank[i] = cc + lower;

@@ -28,6 +28,25 @@ __device__ __forceinline__ void fenceAcquireAgent() { __builtin_amdgcn_fence(__A
 
 __device__ __forceinline__ uint32_t divUpDev(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+// The lanes of the wave whose BITS-bit digit equals this lane's digit d,
+// ANDed with `peers`.  One ballot per digit bit; each ballot is folded into
+// the 64-bit mask with one 3-input bitwise op per half (gfx950
+// v_bitop3_b32 with table 0x90: a & ~(b ^ c)), c being the lane's own bit
+// sign-extended to 0 or all ones, so the mask keeps the ballot where the
+// lane's bit is 1 and its complement where it is 0.
+template <int BITS>
+__device__ __forceinline__ uint64_t matchDigit(uint32_t d, uint64_t peers) {
+  uint32_t lo = (uint32_t)peers, hi = (uint32_t)(peers >> 32);
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
+    const uint64_t m = __ballot(t != 0u);
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)m, t, 0x90);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(m >> 32), t, 0x90);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Adds one to bin `bin` of an LDS histogram for every valid lane of the
 // wave (every lane calls).  When all valid lanes hold the same bin (common
 // for the high digits of spatially coherent keys) one lane adds the count:
