@@ -48,8 +48,9 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
   __shared__ uint32_t s_gstart[kSortRadix];
   __shared__ uint32_t s_scan[WAVES];
   __shared__ uint32_t s_tile;
-  __shared__ K s_keys[TILE];
-  __shared__ uint32_t s_vals[TILE];
+  // staging for the scatter: the keys, then the values in the same words,
+  // so a 64-bit tile takes 32 KiB of LDS and four workgroups fit on a CU
+  __shared__ K s_buf[TILE];
 
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   if (tid == 0) s_tile = atomicAdd(counter, 1u);
@@ -58,14 +59,21 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
   const uint32_t tile = s_tile;
   const uint32_t base = tile * TILE + wave * 64 * ITEMS;
 
+  // the keys now; the values only once the keys are staged (they are not
+  // needed before, and their loads then overlap the look-back)
+  // (a tile inside the array loads without per-item bounds branches)
+  const bool full = (tile + 1) * TILE <= n;
   K key[ITEMS];
-  uint32_t val[ITEMS], rank[ITEMS];
+  uint32_t rank[ITEMS];
+  if (full) {
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    uint32_t idx = base + i * 64 + lane;
-    bool valid = idx < n;
-    key[i] = valid ? keysIn[idx] : (K)0;
-    val[i] = valid ? (valsIn ? valsIn[idx] : idx) : 0u;
+    for (int i = 0; i < ITEMS; ++i) key[i] = keysIn[base + i * 64 + lane];
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t idx = base + i * 64 + lane;
+      key[i] = idx < n ? keysIn[idx] : (K)0;
+    }
   }
 
   // Stable in-wave ranking: item-major order inside a wave's segment equals
@@ -98,8 +106,35 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
   }
   uint32_t *st = status + (size_t)tile * kSortRadix + tid;
   storeRelaxedAgent(st, (tile == 0 ? kFlagInc : kFlagAgg) | total);
-  uint32_t hbase = blockExclusiveScan<kSortBlock>(hist[tid], s_scan);
-  uint32_t bstart = blockExclusiveScan<kSortBlock>(total, s_scan);
+  const uint32_t hbase = blockExclusiveScan<kSortBlock>(hist[tid], s_scan);
+  const uint32_t bstart = blockExclusiveScan<kSortBlock>(total, s_scan);
+  s_bstart[tid] = bstart;
+  __syncthreads();
+
+  // Keys to their tile positions before the look-back (their registers are
+  // free while it waits); rank[i] becomes the item's position in the tile.
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if (base + i * 64 + lane < n) {
+      const uint32_t d = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
+      rank[i] += s_bstart[d] + s_wave[wave][d];
+      s_buf[rank[i]] = key[i];
+    }
+  }
+  uint32_t val[ITEMS];
+  if (!valsIn) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) val[i] = base + i * 64 + lane;
+  } else if (full) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) val[i] = valsIn[base + i * 64 + lane];
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t idx = base + i * 64 + lane;
+      val[i] = idx < n ? valsIn[idx] : 0u;
+    }
+  }
 
   uint32_t excl = 0;
   if (tile > 0) {
@@ -138,30 +173,34 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ k
     }
     storeRelaxedAgent(st, kFlagInc | (excl + total));
   }
-  s_bstart[tid] = bstart;
   s_gstart[tid] = hbase + excl;
   __syncthreads();
 
+  // each staged key gives its position's output index (kept in registers
+  // for the values)
+  const uint32_t tileCount = min((uint32_t)TILE, n - tile * TILE);
+  uint32_t out[ITEMS];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    if (base + i * 64 + lane < n) {
-      uint32_t d = (uint32_t)(key[i] >> shift) & (kSortRadix - 1);
-      uint32_t pos = s_bstart[d] + s_wave[wave][d] + rank[i];
-      s_keys[pos] = key[i];
-      s_vals[pos] = val[i];
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = tid + k * kSortBlock;
+    out[k] = n;  // n marks "nothing to write"
+    if (j < tileCount) {
+      const K kk = s_buf[j];
+      const uint32_t d = (uint32_t)(kk >> shift) & (kSortRadix - 1);
+      out[k] = s_gstart[d] + (j - s_bstart[d]);
+      if (out[k] >= n) out[k] = n;  // only reachable after a bounded-wait error
+      if (keysOut && out[k] < n) keysOut[out[k]] = kk;
     }
   }
   __syncthreads();
-
-  const uint32_t tileCount = min((uint32_t)TILE, n - tile * TILE);
-  for (uint32_t j = tid; j < tileCount; j += kSortBlock) {
-    K k = s_keys[j];
-    uint32_t d = (uint32_t)(k >> shift) & (kSortRadix - 1);
-    uint32_t out = s_gstart[d] + (j - s_bstart[d]);
-    if (out >= n) continue;  // only reachable after a bounded-wait error
-    if (keysOut) keysOut[out] = k;
-    valsOut[out] = s_vals[j];
-  }
+  uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (base + i * 64 + lane < n) s_v[rank[i]] = val[i];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k)
+    if (out[k] < n) valsOut[out[k]] = s_v[tid + k * kSortBlock];
 }
 
 }  // namespace
