@@ -49,29 +49,10 @@ __device__ __forceinline__ uint32_t groupEnd(const uint64_t *kf, uint32_t n, uin
   return lo + 1;
 }
 
-// Each pair of a group of equal top bits (m <= kSplitGroupMax) goes to
-// group start + its rank by (key, index) inside the group (O(m) compares;
-// the lanes of a wave inside one group read the same member at once).  A
-// longer group is listed once, by its first pair, for k_splitLong.
-__global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ kf, const uint32_t *__restrict__ vals,
-                                                   uint32_t n, uint32_t *__restrict__ valsOut,
-                                                   uint64_t *__restrict__ keysOut, uint32_t *__restrict__ ctrl,
-                                                   uint2 *__restrict__ list, uint32_t listCap) {
-  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= n) return;
-  const uint64_t key = kf[p], hi = hiOf(key);
-  const uint32_t v = vals[p];
-  const uint32_t gs = groupStart(kf, p, hi), ge = groupEnd(kf, n, p, hi);
-  if (ge - gs > kSplitGroupMax) {
-    if (p == gs) {
-      const uint32_t slot = atomicAdd(ctrl + 0, 1u);
-      if (slot < listCap)
-        list[slot] = make_uint2(gs, ge - gs);
-      else
-        atomicOr(ctrl + 2, kErrList);
-    }
-    return;
-  }
+// Rank of pair (key, v) inside its group [gs, ge) of the sorted-by-top-bits
+// array, read from global memory (the general path).
+__device__ __forceinline__ uint32_t rankGlobal(const uint64_t *kf, const uint32_t *vals, uint32_t gs, uint32_t ge,
+                                               uint64_t key, uint32_t v) {
   uint32_t rank = 0;
 #pragma unroll 4
   for (uint32_t q = gs; q < ge; ++q) {
@@ -79,8 +60,128 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
     const uint32_t vq = vals[q];
     rank += (kq < key || (kq == key && vq < v)) ? 1u : 0u;
   }
-  valsOut[gs + rank] = v;
-  if (keysOut) keysOut[gs + rank] = key;
+  return rank;
+}
+
+__device__ __forceinline__ uint64_t readLane64(uint64_t x, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int waveMinInt(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ int waveMaxInt(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// Each pair of a group of equal top bits (m <= kSplitGroupMax) goes to
+// group start + its rank by (key, index) inside the group.  A wave owns 64
+// consecutive positions and also loads the 64 before and the 64 after
+// (coalesced; neighbouring waves read them too, from cache).  Group
+// boundaries in those 192 positions are three ballots, so a group that
+// starts and ends inside them is found with bit scans and ranked against
+// its members in an LDS copy of the 192 positions, with no dependent global
+// loads.  A pair whose group reaches past them takes the
+// general path: galloping search for the bounds, members read from global
+// memory (O(m) compares).  A group longer than kSplitGroupMax is listed once,
+// by its first pair, for k_splitLong.
+__global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ kf, const uint32_t *__restrict__ vals,
+                                                   uint32_t n, uint32_t *__restrict__ valsOut,
+                                                   uint64_t *__restrict__ keysOut, uint32_t *__restrict__ ctrl,
+                                                   uint2 *__restrict__ list, uint32_t listCap) {
+  __shared__ uint64_t s_k[4][192];  // per wave: keys of positions P0 - 64 .. P0 + 127
+  __shared__ uint32_t s_v[4][192];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t P0 = blockIdx.x * 256 + (threadIdx.x & ~63u);  // the wave's first position
+  if (P0 >= n) return;                                          // wave-uniform
+  // top bits given to positions below 0 and from n on (never a key's: keys have 63 bits)
+  constexpr uint64_t kBelow = 1ull << 62, kAbove = 1ull << 61;
+  uint64_t k[3], h[3];
+  uint32_t v[3];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const int64_t pos = (int64_t)P0 + 64 * (w - 1) + lane;
+    const bool in = pos >= 0 && pos < (int64_t)n;
+    k[w] = in ? kf[pos] : 0ull;
+    v[w] = in ? vals[pos] : 0u;
+    h[w] = in ? hiOf(k[w]) : (pos < 0 ? kBelow : kAbove);
+  }
+  // bit l of b[w]: a group starts at position P0 + 64 (w - 1) + l.  Whether
+  // one starts at P0 - 64 is left open (0) unless that is position 0.
+  uint64_t b[3];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    // every lane takes part in the shuffle (no cross-lane read inside a
+    // branch); lane 0 then takes the last top bits of the window before
+    const uint64_t up = __shfl_up(h[w], 1, 64);
+    const uint64_t before = w > 0 ? readLane64(h[w - 1], 63) : (P0 == 64 ? kBelow : h[0]);
+    b[w] = __ballot(h[w] != (lane == 0 ? before : up));
+  }
+  const bool valid = P0 + lane < n;
+  const uint64_t key = k[1];
+  const uint32_t val = v[1];
+  // own group [gs, ge) in window coordinates (0 = position P0 - 64); -1: not inside
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // bits 0..lane
+  int gs = -1, ge = -1;
+  if (b[1] & le)
+    gs = 64 + (63 - __clzll((long long)(b[1] & le)));
+  else if (b[0])
+    gs = 63 - __clzll((long long)b[0]);
+  if (b[1] & ~le)
+    ge = 64 + __ffsll((long long)(b[1] & ~le)) - 1;
+  else if (b[2])
+    ge = 128 + __ffsll((long long)b[2]) - 1;
+  else if ((uint64_t)P0 + 128 >= n)
+    ge = (int)(n - (P0 - 64));  // the group runs to the end of the array
+  const bool local = valid && gs >= 0 && ge >= 0;
+
+  // each lane walks its own group's members in the wave's LDS copy of the
+  // 192 positions (lanes of one group read the same words: broadcasts)
+  uint64_t *wk = s_k[threadIdx.x >> 6];
+  uint32_t *wv = s_v[threadIdx.x >> 6];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    wk[64 * w + lane] = k[w];
+    wv[64 * w + lane] = v[w];
+  }
+  const int lenMax = __builtin_amdgcn_readfirstlane(waveMaxInt(local ? ge - gs : 0));
+  uint32_t rank = 0;
+  for (int j = 0; j < lenMax; ++j) {
+    const int q = gs + j;
+    if (local && q < ge) {
+      const uint64_t kq = wk[q];
+      const uint32_t vq = wv[q];
+      rank += (kq < key || (kq == key && vq < val)) ? 1u : 0u;
+    }
+  }
+  if (!valid) return;
+  const uint32_t p = P0 + lane;
+  uint32_t start;
+  if (local) {
+    start = P0 - 64 + (uint32_t)gs;
+  } else {
+    const uint64_t hi = hiOf(key);
+    const uint32_t g0 = groupStart(kf, p, hi), g1 = groupEnd(kf, n, p, hi);
+    if (g1 - g0 > kSplitGroupMax) {
+      if (p == g0) {
+        const uint32_t slot = atomicAdd(ctrl + 0, 1u);
+        if (slot < listCap)
+          list[slot] = make_uint2(g0, g1 - g0);
+        else
+          atomicOr(ctrl + 2, kErrList);
+      }
+      return;
+    }
+    start = g0;
+    rank = rankGlobal(kf, vals, g0, g1, key, val);
+  }
+  valsOut[start + rank] = val;
+  if (keysOut) keysOut[start + rank] = key;
 }
 
 // Long groups: one workgroup per group, a stable LSD over the low 24 bits
