@@ -202,6 +202,25 @@ def run_bench(args):
                 "ms_per_launch": round(dom_ms, 4)}
     print(f"# stages (ms): {json.dumps({k: round(x, 4) for k, x in stages.items()})}", file=sys.stderr, flush=True)
 
+    # per-stage and end-to-end fractions of the HBM peak: modelled bytes over
+    # the stage's measured time (kernels without a byte model add time only)
+    stage_groups = {"encode": ("k_sceneBox", "k_morton"), "sort": SORT_STAGES, "build": ("k_buildLevel",)}
+    stage_roofline = {}
+    total_bytes = 0
+    for g, prefixes in stage_groups.items():
+        names = [k for k in stages if k != "sort_total" and k.startswith(prefixes)]
+        g_ms = sum(stages[k] for k in names)
+        g_bytes = sum(nbytes(k) or 0 for k in names)
+        total_bytes += g_bytes
+        if g_ms > 0:
+            stage_roofline[g] = {"ms": round(g_ms, 4), "bytes": int(g_bytes),
+                                 "gbs": round(g_bytes / (g_ms * 1e-3) / 1e9, 1),
+                                 "frac": round(g_bytes / (g_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    e2e = total_bytes / (ms_per_step * 1e-3) / 1e9
+    stage_roofline["end_to_end"] = {"ms": round(ms_per_step, 4), "bytes": int(total_bytes), "bytes_per_tri": round(total_bytes / n, 1),
+                                    "gbs": round(e2e, 1), "frac": round(e2e / HBM_PEAK_GBS, 4)}
+    print(f"# stage roofline: {json.dumps(stage_roofline)}", file=sys.stderr, flush=True)
+
     # primary + bounce rays
     w, h = 1920, 1080
     nrays = w * h
@@ -272,9 +291,10 @@ def run_bench(args):
            "config": {"workload": "icosphere clusters + triangle soup, LBVH build + refit, 1920x1080 primary + "
                                   "1 diffuse bounce", "triangles": n, "spheres": 2048, "soup": 500000,
                       "morton_bits": bits, "topology": "reference (index Karras)", "seed": 2,
-                      "sort": "onesweep, 8-bit digits" if builder.mem.opts.digitBits == 8 else
-                              "MSD: 16-bit buckets, in-LDS sorts by (key, index)"},
-           "roofline": roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
+                      "sort": ("MSD: 16-bit buckets, in-LDS sorts by (key, index)" if builder.mem.opts.digitBits == 16
+                               else "onesweep over bits 24-62 (8-bit digits), then (key, index) rank in groups of "
+                                    "equal top bits" if bits == 63 else "onesweep, 8-bit digits")},
+           "roofline": roofline, "roofline_stages": stage_roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
            "trace": trace, "bittner": bittner, "batched": batch, "learned_priority": learned, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
@@ -319,7 +339,21 @@ def batch_bench(bt, L, reps=5):
     seq_ms = (time.perf_counter() - t0) * 1e3 / reps
     for lb in builders:
         lb.free()
+    # modelled bytes of the whole batched build (u32 keys, one pass per
+    # digit): mesh boxes 12 n + 12 V; Morton 12 n + 12 V + 4 n; each sort
+    # pass key + value read and written (the first reads no value, the last
+    # writes no key); level-0 build 16 n + 12 V + 48-byte leaves and inner
+    # nodes below the top tiles
+    nv_all = int(vc.sum())
+    passes = 4 if int(b.mem.radixBits) == 8 else 3
+    sort_b = sum(total * ((4 + (0 if p == 0 else 4)) + ((0 if p == passes - 1 else 4) + 4)) for p in range(passes))
+    inner = sum(int(x) - -(-int(x) // (1 << int(best))) for x in fc)
+    batch_bytes = (12 * total + 12 * nv_all) + (16 * total + 12 * nv_all) + sort_b + (16 * total + 12 * nv_all
+                                                                                       + 48 * (total + inner))
+    batch_gbs = batch_bytes / (graph_ms * 1e-3) / 1e9
     out = {"meshes": int(len(fc)), "triangles": total, "tile_log2": int(best), "radix_bits": int(b.mem.radixBits),
+           "roofline_end_to_end": {"bytes_per_tri": round(batch_bytes / total, 1), "gbs": round(batch_gbs, 1),
+                                   "frac": round(batch_gbs / HBM_PEAK_GBS, 4)},
            "autotune_ms": {"radix8_tiles_8_to_11": [round(x, 3) for x in ms[:4]],
                            "radix10_tiles_8_to_11": [round(x, 3) for x in ms[4:]]},
            "graph_ms": round(graph_ms, 3),
