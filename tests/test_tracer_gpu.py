@@ -122,3 +122,26 @@ def test_ray_ordering_does_not_change_hits(bvh, spheres):
     pix1, bh1 = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
     assert (bh0["triId"] != 0xFFFFFFFF).sum() > 1000
     assert bh0.tobytes() == bh1.tobytes() and pix0.tobytes() == pix1.tobytes()
+
+
+def test_trace_full_scene_subsample(bvh, orc):
+    """The ~4M-triangle scene at its full size: 1920x1080 primary rays and one
+    diffuse bounce traced on the GPU; every 16th ray of each set is traced
+    again by the oracle on the same tree and must match bit for bit."""
+    v, f = bvh.scene_spheres_soup(2048, 500000, 2)
+    print(f"full scene n={len(f)}", flush=True)
+    mesh = bvh.DeviceMesh(v, f)
+    tree = bvh.lbvh_build(mesh, 63)
+    nodes = tree.download()
+    rays = bvh.generate_rays(1920, 1080, **BIG_CAM)
+    pix, hits = bvh.trace_rays(tree, mesh, rays)
+    sub = slice(0, None, 16)
+    opix, ohits = orc.trace(np.ascontiguousarray(rays[sub]), nodes, 0, v, f)
+    assert (ohits["triId"] != 0xFFFFFFFF).mean() > 0.5
+    assert np.array_equal(hits[sub], ohits) and pix[sub].tobytes() == opix.tobytes()
+    brays, skip = bvh.bounce_rays(mesh, rays, hits, seed=3)
+    bpix, bh = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
+    obpix, obh = orc.trace(np.ascontiguousarray(brays[sub]), nodes, 0, v, f, mode=1, tmin=0.0,
+                           skip=np.ascontiguousarray(skip[sub]))
+    assert (obh["triId"] != 0xFFFFFFFF).mean() > 0.2
+    assert np.array_equal(bh[sub], obh) and bpix[sub].tobytes() == obpix.tobytes()
