@@ -104,6 +104,8 @@ struct PackedTri {
 };
 constexpr uint32_t kInnerChild = 0xFFFFFFFFu, kLookupLeaf = 0xFFFFFFFEu, kEmptyLeaf = 0xFFFFFFFDu;
 constexpr int kStackCap = 16;  // cached remembered children per lane (power of two)
+constexpr uint32_t kTraceWavesPerCU = 20;  // 160 KiB of LDS / 8 KiB of stack cache per wave
+constexpr uint32_t kRefillAt = 16;         // idle lanes that make a k_traceRaysDyn wave fetch rays
 
 __device__ __forceinline__ uint32_t leafInfo(const bvht_BVHNode &n) {
   if (n.isLeafFlag == 0) return kInnerChild;
@@ -198,95 +200,124 @@ struct TraceCtx {
   uint32_t numFaces;
 };
 
-// kTraceRay traversal loop -- CUDAKernels.cu:146-204, over packed nodes.
-// MODE 0: reference triangle test.  MODE 1: hits need t > tMin and are not
-// the triangle `skip` (secondary rays leaving a surface).  `stk` is this
-// lane's column of a [kStackCap][64] LDS array.
+// kTraceRay traversal loop -- CUDAKernels.cu:146-204, over packed nodes,
+// as a state machine: travInit sets up a walk, each travStep visits one
+// node (an inner node's two child boxes, or a leaf's triangles) and moves
+// to the next.  MODE 0: reference triangle test.  MODE 1: hits need
+// t > tMin and are not the triangle `skip` (secondary rays leaving a
+// surface).  `stk` is this lane's column of a [kStackCap][64] LDS array.
+struct TravState {
+  uint64_t lo, hi;  // bit trail: a 1 marks a level whose second child is still pending
+  uint32_t node, info, top, cached, steps;
+  float nearest;
+  TraceOut o;
+};
+
+// Returns false when there is nothing to walk (the root lies outside the tree).
+__device__ __forceinline__ bool travInit(TravState &s, const TraceCtx &cx, uint32_t root) {
+  s.o = {kNoHit, HUGE_VALF, 0};
+  s.nearest = HUGE_VALF;
+  s.lo = s.hi = 0;
+  s.top = s.cached = s.steps = 0;
+  if (root >= cx.bound) return false;
+  s.node = root;
+  s.info = leafInfo(cx.nodes[root]);
+  return true;
+}
+
+// One visit; returns true when the walk has ended.
+template <int MODE>
+__device__ __forceinline__ bool travStep(TravState &s, const bvht_Ray &ray, const TraceCtx &cx, float tMin,
+                                         uint32_t skip, uint2 *stk) {
+  float dist;
+  if (s.info == kInnerChild) {
+    s.o.intCount++;
+    const PackedPair pp = cx.pairs[s.node];
+    float minL, minR;
+    const bool hl = hitBox6(pp.a.x, pp.a.y, pp.a.z, pp.a.w, pp.b.x, pp.b.y, ray, 0.01f, s.nearest + 0.01f, minL);
+    const bool hr = hitBox6(pp.b.z, pp.b.w, pp.c.x, pp.c.y, pp.c.z, pp.c.w, ray, 0.01f, s.nearest + 0.01f, minR);
+    if (hl || hr) {
+      s.hi = (s.hi << 1) | (s.lo >> 63);
+      s.lo <<= 1;
+      if (hl && hr) {
+        s.lo |= 1;
+        const bool leftFirst = minL < minR;
+        stk[s.top * 64] = leftFirst ? make_uint2(pp.d.y, pp.d.w) : make_uint2(pp.d.x, pp.d.z);
+        s.top = (s.top + 1) & (kStackCap - 1);
+        s.cached = s.cached < kStackCap ? s.cached + 1 : kStackCap;
+        s.node = leftFirst ? pp.d.x : pp.d.y;
+        s.info = leftFirst ? pp.d.z : pp.d.w;
+      } else {
+        s.node = hl ? pp.d.x : pp.d.y;
+        s.info = hl ? pp.d.z : pp.d.w;
+      }
+      return false;
+    }
+  } else if (s.info != kEmptyLeaf) {
+    uint32_t first = s.info, count = 1;
+    if (s.info == kLookupLeaf) {
+      first = cx.nodes[s.node].left;
+      count = cx.nodes[s.node].right;
+    }
+    for (uint32_t k = 0; k < count; ++k) {
+      const uint32_t t = first + k;
+      if (t >= cx.numFaces) break;
+      bool h = hitTriPacked(ray, cx.tris[t], dist);
+      if (MODE == 1) h = h && dist > tMin && t != skip;
+      if (h && dist < s.nearest) {
+        s.nearest = dist;
+        s.o.tri = t;
+      }
+    }
+  }
+  // backtrack to the deepest remembered child
+  if (s.lo == 0 && s.hi == 0) return true;
+  uint32_t tz;
+  if (s.lo != 0) {
+    tz = (uint32_t)__builtin_ctzll(s.lo);
+    if (tz) {
+      s.lo = (s.lo >> tz) | (s.hi << (64 - tz));
+      s.hi >>= tz;
+    }
+  } else {
+    tz = 64 + (uint32_t)__builtin_ctzll(s.hi);
+    s.lo = s.hi >> (tz - 64);
+    s.hi = 0;
+  }
+  s.lo ^= 1;
+  if (s.cached > 0) {
+    s.top = (s.top - 1) & (kStackCap - 1);
+    --s.cached;
+    const uint2 e = stk[s.top * 64];
+    s.node = e.x;
+    s.info = e.y;
+  } else {  // the entry fell out of the cache: walk the parent links
+    uint32_t node = s.node;
+    for (uint32_t k = 0; k < tz && node < cx.bound; ++k) node = cx.nodes[node].parent;
+    if (node >= cx.bound) return true;
+    const uint32_t p = cx.nodes[node].parent;
+    if (p >= cx.bound) return true;
+    node = cx.nodes[node].isLeft == 0 ? cx.nodes[p].left : cx.nodes[p].right;
+    if (node >= cx.bound) return true;
+    s.node = node;
+    s.info = leafInfo(cx.nodes[node]);
+  }
+  return false;
+}
+
+// A valid tree is walked in at most ~4 visits per node; the step bound only
+// ends the walk on a malformed (cyclic) tree.
+constexpr uint32_t kMaxSteps = 1u << 26;
+
 template <int MODE>
 __device__ TraceOut traversePacked(const bvht_Ray &ray, const TraceCtx &cx, uint32_t root, float tMin, uint32_t skip,
                                    uint2 *stk) {
-  TraceOut o = {kNoHit, HUGE_VALF, 0};
-  if (root >= cx.bound) return o;
-  uint64_t lo = 0, hi = 0;
-  uint32_t node = root, info = leafInfo(cx.nodes[root]);
-  uint32_t top = 0, cached = 0;
-  float nearest = HUGE_VALF, dist;
-  // A valid tree is walked in at most ~4 visits per node; the bound only
-  // ends the loop on a malformed (cyclic) tree.
-  for (uint32_t steps = 0; steps < (1u << 26); ++steps) {
-    if (info == kInnerChild) {
-      o.intCount++;
-      const PackedPair pp = cx.pairs[node];
-      float minL, minR;
-      const bool hl = hitBox6(pp.a.x, pp.a.y, pp.a.z, pp.a.w, pp.b.x, pp.b.y, ray, 0.01f, nearest + 0.01f, minL);
-      const bool hr = hitBox6(pp.b.z, pp.b.w, pp.c.x, pp.c.y, pp.c.z, pp.c.w, ray, 0.01f, nearest + 0.01f, minR);
-      if (hl || hr) {
-        hi = (hi << 1) | (lo >> 63);
-        lo <<= 1;
-        if (hl && hr) {
-          lo |= 1;
-          const bool leftFirst = minL < minR;
-          stk[top * 64] = leftFirst ? make_uint2(pp.d.y, pp.d.w) : make_uint2(pp.d.x, pp.d.z);
-          top = (top + 1) & (kStackCap - 1);
-          cached = cached < kStackCap ? cached + 1 : kStackCap;
-          node = leftFirst ? pp.d.x : pp.d.y;
-          info = leftFirst ? pp.d.z : pp.d.w;
-        } else {
-          node = hl ? pp.d.x : pp.d.y;
-          info = hl ? pp.d.z : pp.d.w;
-        }
-        continue;
-      }
-    } else if (info != kEmptyLeaf) {
-      uint32_t first = info, count = 1;
-      if (info == kLookupLeaf) {
-        first = cx.nodes[node].left;
-        count = cx.nodes[node].right;
-      }
-      for (uint32_t k = 0; k < count; ++k) {
-        const uint32_t t = first + k;
-        if (t >= cx.numFaces) break;
-        bool h = hitTriPacked(ray, cx.tris[t], dist);
-        if (MODE == 1) h = h && dist > tMin && t != skip;
-        if (h && dist < nearest) {
-          nearest = dist;
-          o.tri = t;
-        }
-      }
+  TravState s;
+  if (travInit(s, cx, root))
+    while (!travStep<MODE>(s, ray, cx, tMin, skip, stk) && ++s.steps < kMaxSteps) {
     }
-    // backtrack to the deepest remembered child
-    if (lo == 0 && hi == 0) break;
-    uint32_t tz;
-    if (lo != 0) {
-      tz = (uint32_t)__builtin_ctzll(lo);
-      if (tz) {
-        lo = (lo >> tz) | (hi << (64 - tz));
-        hi >>= tz;
-      }
-    } else {
-      tz = 64 + (uint32_t)__builtin_ctzll(hi);
-      lo = hi >> (tz - 64);
-      hi = 0;
-    }
-    lo ^= 1;
-    if (cached > 0) {
-      top = (top - 1) & (kStackCap - 1);
-      --cached;
-      const uint2 e = stk[top * 64];
-      node = e.x;
-      info = e.y;
-    } else {  // the entry fell out of the cache: walk the parent links
-      for (uint32_t k = 0; k < tz && node < cx.bound; ++k) node = cx.nodes[node].parent;
-      if (node >= cx.bound) break;
-      const uint32_t p = cx.nodes[node].parent;
-      if (p >= cx.bound) break;
-      node = cx.nodes[node].isLeft == 0 ? cx.nodes[p].left : cx.nodes[p].right;
-      if (node >= cx.bound) break;
-      info = leafInfo(cx.nodes[node]);
-    }
-  }
-  o.t = nearest;
-  return o;
+  s.o.t = s.nearest;
+  return s.o;
 }
 
 // Shading of CUDAKernels.cu:208-218.
@@ -430,6 +461,62 @@ __global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ r
   if (img) img[i] = shade(o, ray, vert, faces, light);
 }
 
+// Rays of one call traced by resident waves that refill their idle lanes
+// from a shared counter (`next`, zero at launch), so a wave does not idle
+// behind its longest walk.  A wave takes at least kRefillAt rays at a time, in
+// slot order (neighbouring rays stay together); each ray's walk is the
+// same visit sequence as in k_traceRays.  Every wave ends once the counter
+// has passed `num` and its lanes are idle.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_traceRaysDyn(const bvht_Ray *__restrict__ rays, uint32_t num, TraceCtx cx,
+                                                     uint32_t root, const bvht_vec3 *__restrict__ vert,
+                                                     const bvht_Triangle *__restrict__ faces, bvht_vec3 light,
+                                                     float tMin, const uint32_t *__restrict__ skip,
+                                                     bvht_Hit *__restrict__ hits, bvht_CUDAPixel *__restrict__ img,
+                                                     const uint32_t *__restrict__ order, uint32_t *__restrict__ next) {
+  __shared__ uint2 s_stk[kStackCap * 64];
+  uint2 *stk = s_stk + threadIdx.x;
+  const uint64_t lt = __lanemask_lt();
+  bool active = false, drained = false;
+  uint32_t i = 0, skipTri = kNoHit;
+  bvht_Ray ray;
+  TravState s;
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nIdle = (uint32_t)__popcll(idle);
+    if (!drained && (nIdle >= kRefillAt || nIdle == 64)) {  // wave-uniform
+      uint32_t base = 0;
+      if (threadIdx.x == 0) base = atomicAdd(next, nIdle);
+      base = __shfl(base, 0, 64);
+      if (base + nIdle >= num) drained = true;
+      if (!active) {
+        const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
+        if (slot < num) {
+          i = order ? order[slot] : slot;
+          ray = rays[i];
+          skipTri = skip ? skip[i] : kNoHit;
+          active = !(ray.dir.x == 0.0f && ray.dir.y == 0.0f && ray.dir.z == 0.0f) && travInit(s, cx, root);
+          if (!active) {  // nothing to walk: a miss
+            const TraceOut o = {kNoHit, HUGE_VALF, 0};
+            if (hits) hits[i] = bvht_Hit{o.tri, o.t};
+            if (img) img[i] = shade(o, ray, vert, faces, light);
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active && (travStep<MODE>(s, ray, cx, tMin, skipTri, stk) || ++s.steps >= kMaxSteps)) {
+      s.o.t = s.nearest;
+      if (hits) hits[i] = bvht_Hit{s.o.tri, s.o.t};
+      if (img) img[i] = shade(s.o, ray, vert, faces, light);
+      active = false;
+    }
+  }
+}
+
 // kRefitLeafBVHTris (CUDAKernels.cu:486-520): leaf boxes from the current
 // vertices (triangle box -+ FLT_EPSILON).  The leaf's SA is recomputed too
 // (the reference leaves it stale).
@@ -565,8 +652,9 @@ struct TraceCache {
   PackedTri *tris = nullptr;
   size_t triCap = 0;
   uint32_t *bad = nullptr;
+  uint32_t *next = nullptr;  // ray counter of k_traceRaysDyn
   // ray ordering (bvht_traceRays): keys and indices x2, sort scratch
-  bool orderRays = true;
+  bool orderRays = false;  // measured: with lane refills the sort costs more than it saves
   uint32_t orderCap = 0;
   uint32_t *okeys[2] = {nullptr, nullptr}, *ovals[2] = {nullptr, nullptr}, *oscratch = nullptr;
 };
@@ -742,13 +830,20 @@ bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uin
   // secondary rays only: primary rays of an image are coherent in their own order
   const uint32_t *order = nullptr;  // (measured: ordering primary rays costs more than it saves)
   if (mode == 1 && traceCache().orderRays && !(order = rayOrder(rays, num, nodes, rootNode))) return false;
-  if (mode == 0)
+  if (mode == 0) {  // primary rays: one ray per lane, waves in slot order (coherent neighbours)
     k_traceRays<0><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
                                                        skipTri, hits, img, order);
-  else
-    k_traceRays<1><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
-                                                       skipTri, hits, img, order);
-  return checkLaunch("k_traceRays");
+    return checkLaunch("k_traceRays");
+  }
+  // secondary rays: resident waves refill idle lanes (walk lengths vary widely)
+  TraceCache &c = traceCache();
+  if (!c.next && !devAlloc(&c.next, 1, "trace ray counter")) return false;
+  if (!fillU32(c.next, 0u, 1)) return false;
+  // resident waves only: kTraceWavesPerCU of them fill a CU (LDS-bound)
+  const uint32_t waves = std::min<uint32_t>(divUp(num, 64), (uint32_t)numCUs() * kTraceWavesPerCU);
+  k_traceRaysDyn<1><<<waves, 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin, skipTri,
+                                                hits, img, order, c.next);
+  return checkLaunch("k_traceRaysDyn");
 }
 
 void refitDynamicTris(bvht_CUDAMemoryBVHPointer *bvh, bvht_MeshRaw mesh, uint32_t *nodesToUpdate, uint32_t numNodes,

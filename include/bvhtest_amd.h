@@ -458,9 +458,10 @@ void tracerDoCudaSync(void);
  * (and the reference pixel when img != NULL).  mode 0 = reference triangle
  * test (no t-min, CUDAKernels.cu:88-112); mode 1 = t > tMin and the triangle
  * `skipTri[i]` (if skipTri != NULL) is ignored, for secondary rays.
- * Unless switched off with bvht_setRayOrdering(false), mode-1 rays are traced
- * in the order of a sort by (direction cell, origin cell); every ray's
- * result is the same either way, only the memory locality differs. */
+ * Mode-1 rays are traced by resident waves that refill idle lanes from a
+ * shared counter; with bvht_setRayOrdering(true) they are first sorted by
+ * (direction cell, origin cell).  Every ray's result is the same either way,
+ * only the scheduling and memory locality differ. */
 bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
                     bvht_vec3 light, int mode, float tMin, const uint32_t *skipTri, bvht_Hit *hits,
                     bvht_CUDAPixel *img);

@@ -107,19 +107,20 @@ def test_bounce_rays(bvh, orc, spheres):
 
 
 def test_ray_ordering_does_not_change_hits(bvh, spheres):
-    """Secondary rays are traced in a sorted order by default; switching the
-    ordering off must give the same hits and pixels byte for byte."""
+    """Secondary rays traced after a sort by (direction cell, origin cell)
+    (bvht_setRayOrdering(true)) must give the same hits and pixels byte for
+    byte as the default slot order."""
     v, f, mesh, tree, nodes = spheres
     rays = bvh.generate_rays(320, 180, **BIG_CAM)
     _, hits = bvh.trace_rays(tree, mesh, rays)
     brays, skip = bvh.bounce_rays(mesh, rays, hits, seed=11)
     L = bvh.lib()
+    pix0, bh0 = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
     try:
-        L.bvht_setRayOrdering(False)
-        pix0, bh0 = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
-    finally:
         L.bvht_setRayOrdering(True)
-    pix1, bh1 = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
+        pix1, bh1 = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
+    finally:
+        L.bvht_setRayOrdering(False)
     assert (bh0["triId"] != 0xFFFFFFFF).sum() > 1000
     assert bh0.tobytes() == bh1.tobytes() and pix0.tobytes() == pix1.tobytes()
 
