@@ -35,8 +35,10 @@ __global__ __launch_bounds__(kSortBlock) void k_histogram(const K *__restrict__ 
 // ranks its keys stably with wave64 ballots, publishes per-digit counts,
 // looks back over earlier tiles for its global offsets and writes the tile
 // out digit-run by digit-run from LDS.
+// Three workgroups per CU (168 VGPRs; 64-bit tiles spill 13 keys to scratch
+// during the ranking, measured faster than fewer, smaller tiles).
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(kSortBlock) void k_onesweep(const K *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
+__global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
                                                          K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                          uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
                                                          uint32_t *__restrict__ status, uint32_t *__restrict__ counter) {
