@@ -36,7 +36,8 @@ __global__ __launch_bounds__(kSortBlock) void k_histogram(const K *__restrict__ 
 // looks back over earlier tiles for its global offsets and writes the tile
 // out digit-run by digit-run from LDS.
 // Three workgroups per CU (168 VGPRs; 64-bit tiles spill 13 keys to scratch
-// during the ranking, measured faster than fewer, smaller tiles).
+// during the ranking, measured faster than fewer, smaller tiles: 41 vs 45 us
+// per 4.24M-key pass; 32-bit keys at 20 per thread 40 vs 43 us at 15).
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
                                                          K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,

@@ -16,7 +16,7 @@ namespace bvht {
 constexpr int kSortRadixBits = 8;
 constexpr int kSortRadix = 1 << kSortRadixBits;
 constexpr int kSortBlock = 256;
-constexpr int kSortItems32 = 15;  // keys per thread, 32-bit keys
+constexpr int kSortItems32 = 20;  // keys per thread, 32-bit keys
 constexpr int kSortItems64 = 22;  // keys per thread, 64-bit keys: 753 tiles at 4.24M keys, one wave of workgroups
 
 struct SortPlan {
