@@ -45,11 +45,23 @@ def cases(bits):
              for b in rng.integers(0, 1 << 16, 12)]
     c["clusters"] = np.concatenate(blobs + [rng.integers(0, hi, 100000, dtype=np.uint64)])
     rng.shuffle(c["clusters"])
+    # runs of equal high bits (24 and up for 63-bit keys) of many lengths,
+    # shuffled: the 63-bit group step's groups inside one wave's positions,
+    # across its neighbours', past them (galloping search) and longer than
+    # 4096 (the long-group sort)
+    lens = np.concatenate([rng.integers(1, 64, 300), rng.integers(64, 193, 60), rng.integers(193, 4097, 12),
+                           [4096, 4097, 5000]])
+    shift = 24 if bits == 63 else 14
+    grp = np.repeat(np.arange(len(lens), dtype=np.uint64), lens) << np.uint64(shift)
+    mg = grp | rng.integers(0, 1 << shift, len(grp), dtype=np.uint64)
+    rng.shuffle(mg)
+    c["mixed_groups"] = mg
     return c
 
 
 CASES = ["one", "two_equal", "random_3000", "random_1M", "all_equal_20k", "few_distinct", "one_bucket_spread",
-         "one_bucket_narrow", "window_sized_buckets", "lds_limit_buckets", "sorted", "reversed", "clusters"]
+         "one_bucket_narrow", "window_sized_buckets", "lds_limit_buckets", "sorted", "reversed", "clusters",
+         "mixed_groups"]
 
 
 @pytest.mark.parametrize("digits", [16, 8], ids=["msd", "onesweep"])

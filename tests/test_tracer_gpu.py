@@ -146,3 +146,18 @@ def test_trace_full_scene_subsample(bvh, orc):
                            skip=np.ascontiguousarray(skip[sub]))
     assert (obh["triId"] != 0xFFFFFFFF).mean() > 0.2
     assert np.array_equal(bh[sub], obh) and bpix[sub].tobytes() == obpix.tobytes()
+
+
+@pytest.mark.parametrize("count", [1, 15, 16, 17, 63, 64, 65, 1000, 4097])
+def test_secondary_ray_counts(bvh, orc, spheres, count):
+    """Secondary rays go through lane refills from a shared counter: any
+    number of rays, including fewer than one refill, must match the oracle."""
+    v, f, mesh, tree, nodes = spheres
+    rays = bvh.generate_rays(128, 72, **BIG_CAM)
+    _, hits = bvh.trace_rays(tree, mesh, rays)
+    brays, skip = bvh.bounce_rays(mesh, rays, hits, seed=5)
+    brays = np.ascontiguousarray(brays[:count])
+    skip = np.ascontiguousarray(skip[:count])
+    pix, bh = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
+    opix, obh = orc.trace(brays, nodes, 0, v, f, mode=1, tmin=0.0, skip=skip)
+    assert np.array_equal(bh, obh) and pix.tobytes() == opix.tobytes()
