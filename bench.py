@@ -92,7 +92,7 @@ def profile_build(args):
 
 
 def profile_batch(args):
-    """Kernel-trace runs of the batched build: the 512-mesh C3 batch built
+    """Kernel-trace runs of the batched build: the 512-mesh batch built
     `steps` times without graph capture (tile 2^10 and 8-bit digits, the
     autotuner's choice on MI355X), so every kernel shows up under its own
     name."""
@@ -443,8 +443,8 @@ def main():
     ap.add_argument("--profile-batch", action="store_true", help="batched builds only (kernel-trace runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
-              f"in DESIGN.md and is not run", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); runs on more GPUs are not "
+              f"implemented", file=sys.stderr)
         raise SystemExit(2)
     run_bench(args)
 

@@ -605,7 +605,8 @@ __device__ __forceinline__ uint32_t cell9(float x, float lo, float inv) {
 // Ordering key of a secondary ray: its direction cell (4 bins per axis over
 // [-1, 1]) above the Morton cell of its origin in the root box (8 bits per
 // axis).  Rays that start close together and head the same way walk similar
-// nodes, so a wave of them diverges less (measured at C1: bounce trace 5.6 ->
+// nodes, so a wave of them diverges less (measured on the 4.24M-triangle scene
+// at 1080p, one-ray-per-lane tracing: bounce trace 5.6 ->
 // 4.9 ms with the sort included; octant-only direction cells 4.9, origin-major
 // keys 5.2).  A ray with no direction gets the largest key.  Plus the digit
 // histograms of the 4 sort passes.

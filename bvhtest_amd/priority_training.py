@@ -1,8 +1,8 @@
 """Learned Bittner13 removal priority: records, training and evaluation.
 
 Records come from the plain GPU optimizer itself (1% of the nodes per step,
-32 chunks, the reference defaults) on 32 seeded scenes of the C1 kind
-(icosphere clusters + triangle soup, ~1M triangles each).  Every patch a step evaluates is recorded as the
+32 chunks, the reference defaults) on 32 seeded scenes of icosphere
+clusters + triangle soup (~1M triangles each).  Every patch a step evaluates is recorded as the
 features of its node at selection time and the SAH gain the searches found for
 it.  Each step also probes uniformly sampled inner nodes, so the model sees
 nodes the heuristic would not have picked.  The regression target is an
@@ -104,7 +104,7 @@ def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=
     """Plain: `plain_steps` steps -> SAH S_p in time T_p.  Learned: steps until
     SAH <= 1.005 S_p (at most max_factor * plain_steps) -> T_l.  Step times
     include selection (features + inference for learned) and are measured
-    with a device sync per step; the SAH evaluations that judge each step
+    with a device sync per step; the SAH evaluations that check each step
     are not timed."""
     v, f = scene(seed, spheres, soup)
     mesh = DeviceMesh(v, f)
