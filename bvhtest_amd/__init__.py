@@ -167,12 +167,14 @@ def download_u64(dev_ptr, count):
 class LBVHBuilder:
     """Working memory of the LBVH command (lib/builder/LBVH.cpp:28-55)."""
 
-    def __init__(self, mesh, morton_bits=30, topology=0, digit_bits=0):
+    def __init__(self, mesh, morton_bits=30, topology=0, digit_bits=0, tile_log2=0):
         self.mesh = mesh
         opts = lib().bvht_LBVH_defaultOptions()
         opts.mortonBits = morton_bits
         opts.topology = topology
         opts.digitBits = digit_bits
+        if tile_log2:
+            opts.tileLog2 = tile_log2
         self.mem = lib().bvht_LBVH_allocateWorkingMemoryEx(ctypes.byref(mesh.raw), ctypes.byref(opts))
         if not self.mem.lRes:
             _fail("LBVH_allocateWorkingMemory")
