@@ -68,11 +68,6 @@ __device__ __forceinline__ uint64_t readLane64(uint64_t x, int l) {
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
   return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ int waveMinInt(int x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
-  return x;
-}
 __device__ __forceinline__ int waveMaxInt(int x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
