@@ -448,11 +448,20 @@ __global__ __launch_bounds__(64) void k_traceRays(const bvht_Ray *__restrict__ r
                                                   uint32_t root, const bvht_vec3 *__restrict__ vert,
                                                   const bvht_Triangle *__restrict__ faces, bvht_vec3 light, float tMin,
                                                   const uint32_t *__restrict__ skip, bvht_Hit *__restrict__ hits,
-                                                  bvht_CUDAPixel *__restrict__ img, const uint32_t *__restrict__ order) {
+                                                  bvht_CUDAPixel *__restrict__ img, const uint32_t *__restrict__ order,
+                                                  uint32_t imgW) {
   __shared__ uint2 s_stk[kStackCap * 64];
-  const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
-  if (slot >= num) return;
-  const uint32_t i = order ? order[slot] : slot;
+  uint32_t i;
+  if (imgW) {  // the rays of an imgW-wide image, row-major: one 8x8 pixel tile per wave
+    const uint32_t tilesX = (imgW + 7) / 8;
+    const uint32_t x = (blockIdx.x % tilesX) * 8 + (threadIdx.x & 7), y = (blockIdx.x / tilesX) * 8 + (threadIdx.x >> 3);
+    if (x >= imgW || (uint64_t)y * imgW + x >= num) return;
+    i = y * imgW + x;
+  } else {
+    const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
+    if (slot >= num) return;
+    i = order ? order[slot] : slot;
+  }
   const bvht_Ray ray = rays[i];
   TraceOut o = {kNoHit, HUGE_VALF, 0};
   if (!(ray.dir.x == 0.0f && ray.dir.y == 0.0f && ray.dir.z == 0.0f))
@@ -654,6 +663,9 @@ struct TraceCache {
   size_t triCap = 0;
   uint32_t *bad = nullptr;
   uint32_t *next = nullptr;  // ray counter of k_traceRaysDyn
+  // the ray buffer generateRays last filled, as a w x h image
+  const void *imgRays = nullptr;
+  uint32_t imgW = 0, imgH = 0;
   // ray ordering (bvht_traceRays): keys and indices x2, sort scratch
   bool orderRays = false;  // measured: with lane refills the sort costs more than it saves
   uint32_t orderCap = 0;
@@ -781,7 +793,11 @@ void generateRays(bvht_Ray *rays, uint32_t w, uint32_t h, bvht_vec3 pos, bvht_ve
   c.scale = (float)std::tan((0.5 * (double)fov) * (M_PI / 180.0));  // tan(radians(0.5 * fov))
   dim3 grid(divUp(w, 16), divUp(h, 16));
   k_genRays<<<grid, 256, 0, stream()>>>(rays, w, h, c);
-  checkLaunch("k_genRays");
+  if (!checkLaunch("k_genRays")) return;
+  TraceCache &tc = traceCache();
+  tc.imgRays = rays;
+  tc.imgW = w;
+  tc.imgH = h;
 }
 
 void tracerImage(bvht_Ray *rays, bvht_CUDAPixel *img, bvht_BVHNode *nodes, uint32_t rootNode, bvht_MeshRaw mesh,
@@ -831,9 +847,15 @@ bool bvht_traceRays(const bvht_Ray *rays, uint32_t num, bvht_BVHNode *nodes, uin
   // secondary rays only: primary rays of an image are coherent in their own order
   const uint32_t *order = nullptr;  // (measured: ordering primary rays costs more than it saves)
   if (mode == 1 && traceCache().orderRays && !(order = rayOrder(rays, num, nodes, rootNode))) return false;
-  if (mode == 0) {  // primary rays: one ray per lane, waves in slot order (coherent neighbours)
-    k_traceRays<0><<<divUp(num, 64), 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin,
-                                                       skipTri, hits, img, order);
+  if (mode == 0) {  // primary rays: one ray per lane, neighbouring rays in a wave
+    // Rays that generateRays wrote as an image are traced in 8x8 pixel tiles
+    // (tighter bundles than 64-pixel rows); the order never changes a ray's
+    // result, only which rays share a wave.
+    const TraceCache &c = traceCache();
+    const bool image = rays == c.imgRays && (uint64_t)num == (uint64_t)c.imgW * c.imgH;
+    const uint32_t blocks = image ? divUp(c.imgW, 8) * divUp(c.imgH, 8) : divUp(num, 64);
+    k_traceRays<0><<<blocks, 64, 0, stream()>>>(rays, num, cx, rootNode, mesh.vert, mesh.faces, light, tMin, skipTri,
+                                                hits, img, order, image ? c.imgW : 0u);
     return checkLaunch("k_traceRays");
   }
   // secondary rays: resident waves refill idle lanes (walk lengths vary widely)

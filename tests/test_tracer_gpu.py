@@ -161,3 +161,23 @@ def test_secondary_ray_counts(bvh, orc, spheres, count):
     pix, bh = bvh.trace_rays(tree, mesh, brays, mode=1, tmin=0.0, skip=skip)
     opix, obh = orc.trace(brays, nodes, 0, v, f, mode=1, tmin=0.0, skip=skip)
     assert np.array_equal(bh, obh) and pix.tobytes() == opix.tobytes()
+
+
+@pytest.mark.parametrize("w,h", [(61, 37), (64, 48), (1, 1)])
+def test_image_rays_traced_in_tiles(bvh, orc, spheres, w, h):
+    """Primary rays that generateRays wrote as an image are traced in 8x8
+    pixel tiles (edge tiles partly empty); hits and pixels must equal the
+    oracle's, ray by ray."""
+    v, f, mesh, tree, nodes = spheres
+    L = bvh.lib()
+    rays_dev = bvh.generate_rays_device(w, h, **BIG_CAM)
+    n = w * h
+    hb = bvh.DeviceBuffer(n * bvh.HIT_DTYPE.itemsize)
+    pb = bvh.DeviceBuffer(n * bvh.PIXEL_DTYPE.itemsize)
+    assert L.bvht_traceRays(rays_dev.ptr, n, tree.ptr.nodes, 0, mesh.raw, bvh.vec3((2.0, 2.0, 2.0)), 0, 0.0, None,
+                            hb.ptr, pb.ptr)
+    L.tracerDoCudaSync()
+    hits, pix = hb.download(bvh.HIT_DTYPE, n), pb.download(bvh.PIXEL_DTYPE, n)
+    rays = rays_dev.download(bvh.RAY_DTYPE, n)
+    opix, ohits = orc.trace(rays, nodes, 0, v, f)
+    assert np.array_equal(hits, ohits) and pix.tobytes() == opix.tobytes()
