@@ -181,3 +181,29 @@ def test_image_rays_traced_in_tiles(bvh, orc, spheres, w, h):
     rays = rays_dev.download(bvh.RAY_DTYPE, n)
     opix, ohits = orc.trace(rays, nodes, 0, v, f)
     assert np.array_equal(hits, ohits) and pix.tobytes() == opix.tobytes()
+
+
+@pytest.mark.parametrize("w,h", [(61, 37), (128, 72)])
+def test_image_bounce_rays_on_device_buffers(bvh, orc, spheres, w, h):
+    """A frame as the benchmark runs it, on device buffers: image rays traced
+    in 8x8 tiles, bvht_makeBounceRays, secondary rays through lane refills;
+    every secondary ray must match the oracle."""
+    v, f, mesh, tree, nodes = spheres
+    L = bvh.lib()
+    n = w * h
+    light = bvh.vec3((2.0, 2.0, 2.0))
+    rays_dev = bvh.generate_rays_device(w, h, **BIG_CAM)
+    hb = bvh.DeviceBuffer(n * bvh.HIT_DTYPE.itemsize)
+    assert L.bvht_traceRays(rays_dev.ptr, n, tree.ptr.nodes, 0, mesh.raw, light, 0, 0.0, None, hb.ptr, None)
+    br = bvh.DeviceBuffer(n * bvh.RAY_DTYPE.itemsize)
+    sk = bvh.DeviceBuffer(n * 4)
+    assert L.bvht_makeBounceRays(rays_dev.ptr, hb.ptr, n, mesh.raw, 9, br.ptr, sk.ptr)
+    bh = bvh.DeviceBuffer(n * bvh.HIT_DTYPE.itemsize)
+    pb = bvh.DeviceBuffer(n * bvh.PIXEL_DTYPE.itemsize)
+    assert L.bvht_traceRays(br.ptr, n, tree.ptr.nodes, 0, mesh.raw, light, 1, 0.0, sk.ptr, bh.ptr, pb.ptr)
+    L.tracerDoCudaSync()
+    brays, skip = br.download(bvh.RAY_DTYPE, n), sk.download(np.uint32, n)
+    hits, pix = bh.download(bvh.HIT_DTYPE, n), pb.download(bvh.PIXEL_DTYPE, n)
+    opix, ohits = orc.trace(brays, nodes, 0, v, f, mode=1, tmin=0.0, skip=skip)
+    assert (ohits["triId"] != 0xFFFFFFFF).sum() > 0
+    assert np.array_equal(hits, ohits) and pix.tobytes() == opix.tobytes()
