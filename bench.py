@@ -43,8 +43,9 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
     write; MSD scatter: key read, key + index written; onesweep pass: key +
     value read and written (first pass reads no value, last pass writes no
     key; 63-bit keys: the 5 passes over bits 24-62, all writing keys); group
-    rank: key 8 + index 4 read and written (each group member read once per
-    group, in cache); level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
+    rank: key 8 + index 4 read, index 4 written (each group member read once
+    per group, in cache; the index-based topology built here needs no sorted
+    keys, so none are written); level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
     48-byte leaves (n) and the inner nodes below the top tiles
     (n - ceil(n / 2^tile_log2)) written.  The MSD window and big-bucket sorts
     move a data-dependent share of the pairs, so they have no byte model."""
@@ -61,7 +62,7 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
         wr = (0 if pass_idx == passes - 1 and not keep_last else kb) + 4
         return n * (rd + wr)
     if name == "k_splitRank":
-        return 2 * (8 * n + 4 * n)
+        return 8 * n + 4 * n + 4 * n
     if name == "k_buildLevel0":
         inner = n - -(-n // (1 << tile_log2))
         return 16 * n + 12 * nv + 48 * (n + inner)

@@ -390,7 +390,7 @@ bool runSort(bvht_LBVH_WorkingMemory *mem) {
   }
   if (in->useSplit) {
     uint64_t *k2[2] = {(uint64_t *)mem->keys[0], (uint64_t *)mem->keys[1]};
-    const int r = splitRun(in->split, k2, mem->values, (uint32_t *)mem->sortScratch,
+    const int r = splitRun(in->split, k2, mem->values, (uint32_t *)mem->sortScratch, keepKeys,
                            tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr,
                            tm.enabled ? [](const char *s) { timing().record(s); } : nullptr);
     if (r < 0) return false;

@@ -296,7 +296,8 @@ bool splitReset(const SplitPlan &p, uint32_t *scratch) {
   return sortReset(p.plan, scratch) && fillU32(scratch + p.ctrlOff, 0u, 16);
 }
 
-int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch, void (*afterPass)(int),
+int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch, bool keepKeys,
+             void (*afterPass)(int),
              void (*afterStage)(const char *)) {
   if (p.n == 0) return 0;
   void *kv[2] = {keys[0], keys[1]};
@@ -304,8 +305,8 @@ int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t 
   if (r < 0) return -1;
   uint32_t *ctrl = scratch + p.ctrlOff;
   uint2 *list = reinterpret_cast<uint2 *>(scratch + p.listOff);
-  k_splitRank<<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], vals[r], p.n, vals[r ^ 1], keys[r ^ 1], ctrl, list,
-                                                     p.listCap);
+  k_splitRank<<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], vals[r], p.n, vals[r ^ 1],
+                                                     keepKeys ? keys[r ^ 1] : nullptr, ctrl, list, p.listCap);
   if (!checkLaunch("k_splitRank")) return -1;
   if (afterStage) afterStage("k_splitRank");
   // long groups are rare (none at the 4.24M-triangle scene): a small grid

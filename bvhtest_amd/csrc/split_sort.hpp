@@ -42,8 +42,9 @@ bool splitReset(const SplitPlan &p, uint32_t *scratch);
 
 // keys[0] / vals: the n keys in input order (vals unused on entry: the
 // input index is the value).  Returns the buffer index (0/1) of keys and
-// vals holding the sorted pairs (-1 on a launch error).
-int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch,
+// vals holding the sorted pairs (-1 on a launch error); with keepKeys false
+// only the values there are complete.
+int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch, bool keepKeys,
              void (*afterPass)(int) = nullptr, void (*afterStage)(const char *) = nullptr);
 
 }  // namespace bvht
