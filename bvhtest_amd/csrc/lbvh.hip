@@ -9,10 +9,11 @@
 //                global atomic per digit per block).
 //   onesweep     stable radix sort of (key, face index); the last pass keeps
 //                only the face indices when the topology does not need keys.
-//   k_buildLevel tiles of 2^kTileLog units: level 0 gathers each sorted
-//                triangle, writes the leaf, then reduces the tile's subtree
-//                in LDS and writes each inner node once (box, topology, SA);
-//                further launches reduce tile roots the same way.
+//   k_buildLevel0 tiles of 2^kTileLog positions: gathers each sorted
+//                triangle, writes the leaf, reduces the tile's subtree
+//                (wave shuffles, then LDS) and writes each inner node once
+//                (box, topology, SA) from the lane of its index;
+//   k_buildLevelUp further launches reduce the tile roots in LDS.
 // The reference topology is a closed form of n (see types.hpp), so no key is
 // read to build it and no global atomic is needed for the refit.
 #include "lbvh.hpp"
@@ -440,12 +441,10 @@ static bool treeLevelsT(bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_
     const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
     const uint32_t blocks = (uint32_t)((U + (1u << TLOG) - 1) >> TLOG);
     if (first)
-      k_buildLevel<TLOG, true, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, sd, nullptr,
-                                                                              nullptr, 0);
+      k_buildLevel0<TLOG, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, sd, nullptr, nullptr);
     else
-      k_buildLevel<TLOG, false, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, nullptr, faces, vert, sd, nullptr,
-                                                                               nullptr, unitLog);
-    if (!checkLaunch("k_buildLevel")) return false;
+      k_buildLevelUp<TLOG, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sd, nullptr, nullptr, unitLog);
+    if (!checkLaunch(first ? "k_buildLevel0" : "k_buildLevelUp")) return false;
     if (afterLevel) afterLevel(first);
     first = false;
     if (blocks <= 1) break;
@@ -474,12 +473,12 @@ static bool treeLevelsBatchT(bvht_BVHNode *nodes, const uint32_t *sorted, const 
   for (int L = 0; L < levels; ++L) {
     if (counts[L] == 0) continue;
     if (L == 0)
-      k_buildLevel<TLOG, true, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, none, segs,
-                                                                                tiles[L], 0);
+      k_buildLevel0<TLOG, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, none, segs,
+                                                                           tiles[L]);
     else
-      k_buildLevel<TLOG, false, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, nullptr, faces, vert, none, segs,
-                                                                                 tiles[L], (uint32_t)(L * TLOG));
-    if (!checkLaunch("k_buildLevel batch")) return false;
+      k_buildLevelUp<TLOG, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, none, segs, tiles[L],
+                                                                            (uint32_t)(L * TLOG));
+    if (!checkLaunch(L == 0 ? "k_buildLevel0 batch" : "k_buildLevelUp batch")) return false;
   }
   return true;
 }

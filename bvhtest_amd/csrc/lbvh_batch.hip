@@ -14,7 +14,8 @@
 //                mesh, the look-back stops at the mesh's first tile, and
 //                digit offsets start at the mesh's prefix; u32 keys, 8- or
 //                10-bit digits (4 or 3 passes, chosen by the autotuner);
-//   k_buildLevel the tile hierarchy per mesh (tiles never cross meshes).
+//   k_buildLevel0 / k_buildLevelUp the tile hierarchy per mesh (tiles
+//                never cross meshes).
 #include "lbvh_kernels.hpp"
 
 #include <cmath>

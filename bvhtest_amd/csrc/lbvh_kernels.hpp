@@ -51,18 +51,209 @@ __device__ __forceinline__ float o2f(uint32_t o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-// One level of the tile hierarchy.  Units are aligned runs of 2^unitLog leaf
-// positions of one segment; a workgroup owns 2^TLOG consecutive units and
-// writes every node whose leaf range is a union of 2..2^TLOG of its units
-// (level 0 also gathers each sorted triangle and writes its leaf).  Boxes
-// are reduced level by level in LDS: no global atomics.
-template <int TLOG, bool LEVEL0, bool BATCH>
-__global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__restrict__ nodesAll,
-                                                            const uint32_t *__restrict__ sorted,
-                                                            const bvht_Triangle *__restrict__ faces,
-                                                            const bvht_vec3 *__restrict__ vertAll, SegDesc single,
-                                                            const SegDesc *__restrict__ segs,
-                                                            const uint2 *__restrict__ tiles, uint32_t unitLog) {
+// Inner node whose index is position p of a segment with N1 + 1 leaves
+// (p < N1), in the closed form of the reference topology: an odd p is the
+// last position of a left child (a full aligned block), an even p > 0 the
+// first position of a right child (an aligned block cut at N1), p = 0 the
+// root.  Returns the node's range [l, r].
+__device__ __forceinline__ void nodeAtIndex(uint32_t p, uint32_t N1, uint32_t &l, uint32_t &r) {
+  if (p == 0) {
+    l = 0;
+    r = N1;
+  } else if (p & 1u) {
+    l = p + 1 - (1u << __builtin_ctz(p + 1));
+    r = p;
+  } else {
+    l = p;
+    r = (uint32_t)min((uint64_t)p + (1ull << __builtin_ctz(p)) - 1, (uint64_t)N1);
+  }
+}
+
+// One butterfly step of the in-wave box reduction: b holds the box of the
+// aligned block of `half` leaves around the lane's leaf k and becomes that
+// of the block of 2 * half.  The lower half always comes first in the
+// merge, as a node merges its left child with its right; an upper half
+// starting at or past cnt (no leaves) leaves the box unchanged.
+__device__ __forceinline__ void boxStep(float b[6], uint32_t k, uint32_t half, uint32_t cnt) {
+  float o[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) o[c] = __shfl_xor(b[c], (int)half);
+  if (((k & ~(2 * half - 1)) | half) >= cnt) return;
+  const bool upper = (k & half) != 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float lo = upper ? o[c] : b[c], hi = upper ? b[c] : o[c];
+    b[c] = mergeMin(lo, hi);
+  }
+#pragma unroll
+  for (int c = 3; c < 6; ++c) {
+    const float lo = upper ? o[c] : b[c], hi = upper ? b[c] : o[c];
+    b[c] = mergeMax(lo, hi);
+  }
+}
+
+__device__ __forceinline__ void storeInner(bvht_BVHNode *nodes, uint32_t l, uint32_t r, uint32_t N1, const float b[6]) {
+  bvht_AABB box;
+  box.minMax[0] = {b[0], b[1], b[2]};
+  box.minMax[1] = {b[3], b[4], b[5]};
+  const uint32_t m = rangeSplit(l, r);
+  uint32_t parent, isLeft;
+  parentOf(l, r, N1, parent, isLeft);
+  storeNode(nodes + rangeNodeIndex(l, r, N1), box, parent, rangeNodeIndex(l, m - 1, N1), rangeNodeIndex(m, r, N1), 0u,
+            isLeft, 0u, surfaceArea(box));
+}
+
+// Level 0 of the tile hierarchy: a workgroup owns 2^TLOG consecutive sorted
+// positions of one segment (a tile), gathers each sorted triangle, writes
+// its leaf, and writes every inner node whose range lies in the tile.  The
+// inner node written by the lane of position p is the one whose index is p,
+// so a wave's inner nodes are as contiguous as its leaves (an index of a
+// node in the tile lies in the tile).  Boxes of aligned blocks of up to 64
+// positions come from butterfly shuffles inside the wave; the few larger
+// blocks of the tile are reduced in LDS from the 64-position blocks.
+template <int TLOG, bool BATCH>
+__global__ __launch_bounds__(kBuildBlock) void k_buildLevel0(bvht_BVHNode *__restrict__ nodesAll,
+                                                             const uint32_t *__restrict__ sorted,
+                                                             const bvht_Triangle *__restrict__ faces,
+                                                             const bvht_vec3 *__restrict__ vertAll, SegDesc single,
+                                                             const SegDesc *__restrict__ segs,
+                                                             const uint2 *__restrict__ tiles) {
+  constexpr uint32_t T = 1u << TLOG;
+  constexpr int R = T / kBuildBlock;
+  constexpr uint32_t NB = T / 64;  // 64-position blocks of a tile
+  static_assert(T % kBuildBlock == 0 && TLOG > 6, "tile holds whole workgroups of 64-position blocks");
+  // block boxes of levels 6..TLOG; level j starts at 2 NB - (2 NB >> (j - 6))
+  __shared__ float s_box[2 * NB][6];
+  SegDesc sd;
+  uint32_t group;
+  if (BATCH) {
+    const uint2 t = tiles[blockIdx.x];
+    sd = segs[t.x];
+    group = t.y;
+  } else {
+    sd = single;
+    group = blockIdx.x;
+  }
+  const uint32_t n = sd.n, N1 = n - 1;
+  bvht_BVHNode *__restrict__ nodes = nodesAll + sd.nodeBase;
+  const bvht_vec3 *__restrict__ vert = vertAll + sd.vertBase;
+  const uint32_t u0 = group << TLOG;
+  if (u0 >= n) return;  // block-uniform: a group past the last position owns nothing
+  const uint32_t cnt = min(T, n - u0);
+
+  // Every sorted index, face and vertex gather of the thread's leaves is
+  // issued before any is used: one chain of dependent loads per thread
+  // instead of one per leaf.
+  uint32_t f[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint32_t k = threadIdx.x + i * kBuildBlock;
+    f[i] = k < cnt ? sorted[sd.faceBase + u0 + k] - sd.faceBase : 0u;
+    if (f[i] >= n) f[i] = 0;  // never read outside the mesh, even from a corrupt permutation
+  }
+  bvht_Triangle t[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) t[i] = faces[sd.faceBase + f[i]];
+  bvht_vec3 va[R], vb[R], vc[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    va[i] = vert[t[i].v1];
+    vb[i] = vert[t[i].v2];
+    vc[i] = vert[t[i].v3];
+  }
+  // leaves first: the gathered vertices are dead before the reductions start
+  float leaf[R][6];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint32_t k = threadIdx.x + i * kBuildBlock, p = u0 + k;
+    const bvht_AABB lb = triangleBox(va[i], vb[i], vc[i]);
+    if (k < cnt) {
+      uint32_t parent, isLeft;
+      parentOf(p, p, N1, parent, isLeft);
+      // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
+      storeNode(nodes + N1 + p, lb, parent, f[i], 1u, 1u, isLeft, 255u, surfaceArea(lb));
+    }
+    leaf[i][0] = lb.minMax[0].x;
+    leaf[i][1] = lb.minMax[0].y;
+    leaf[i][2] = lb.minMax[0].z;
+    leaf[i][3] = lb.minMax[1].x;
+    leaf[i][4] = lb.minMax[1].y;
+    leaf[i][5] = lb.minMax[1].z;
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint32_t k = threadIdx.x + i * kBuildBlock, p = u0 + k;
+    // the inner node with index p, if it lies in the tile
+    uint32_t l = 0, r = 0, jn = 0;
+    if (k < cnt && p < N1) {
+      nodeAtIndex(p, N1, l, r);
+      if (l >= u0 && r < u0 + cnt) jn = 32 - __builtin_clz(l ^ r);
+    }
+    float *b = leaf[i];
+    float pick[6];
+#pragma unroll
+    for (uint32_t j = 1; j <= 6; ++j) {
+      boxStep(b, k, 1u << (j - 1), cnt);
+      if (jn == j) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) pick[c] = b[c];
+      }
+    }
+    if ((k & 63) == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_box[k >> 6][c] = b[c];
+    }
+    if (jn >= 1 && jn <= 6) storeInner(nodes, l, r, N1, pick);
+  }
+  __syncthreads();
+  // levels 7..TLOG: block a of level j merges blocks 2a and 2a + 1 of level j - 1
+#pragma unroll
+  for (uint32_t j = 7; j <= (uint32_t)TLOG; ++j) {
+    const uint32_t src = 2 * NB - ((2 * NB) >> (j - 7)), dst = 2 * NB - ((2 * NB) >> (j - 6));
+    const uint32_t a = threadIdx.x;
+    if (a < (NB >> (j - 6))) {
+      float b[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) b[c] = s_box[src + 2 * a][c];
+      if ((2 * a + 1) << (j - 1) < cnt) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) b[c] = mergeMin(b[c], s_box[src + 2 * a + 1][c]);
+#pragma unroll
+        for (int c = 3; c < 6; ++c) b[c] = mergeMax(b[c], s_box[src + 2 * a + 1][c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_box[dst + a][c] = b[c];
+    }
+    __syncthreads();
+  }
+  // inner nodes of blocks larger than a wave's 64 positions (a few per tile)
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const uint32_t k = threadIdx.x + i * kBuildBlock, p = u0 + k;
+    if (k >= cnt || p >= N1) continue;
+    uint32_t l, r;
+    nodeAtIndex(p, N1, l, r);
+    if (l < u0 || r >= u0 + cnt) continue;
+    const uint32_t jn = 32 - __builtin_clz(l ^ r);
+    if (jn <= 6) continue;
+    const uint32_t at = 2 * NB - ((2 * NB) >> (jn - 6)) + (k >> jn);
+    float b[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) b[c] = s_box[at][c];
+    storeInner(nodes, l, r, N1, b);
+  }
+}
+
+// One upper level of the tile hierarchy.  Units are aligned runs of
+// 2^unitLog leaf positions of one segment (unitLog a multiple of TLOG, at
+// least TLOG); a workgroup owns 2^TLOG consecutive units and writes every
+// node whose leaf range is a union of 2..2^TLOG of its units, reading the
+// units' boxes from their nodes.  Boxes are reduced level by level in LDS:
+// no global atomics.
+template <int TLOG, bool BATCH>
+__global__ __launch_bounds__(kBuildBlock) void k_buildLevelUp(bvht_BVHNode *__restrict__ nodesAll,
+                                                              const SegDesc single, const SegDesc *__restrict__ segs,
+                                                              const uint2 *__restrict__ tiles, uint32_t unitLog) {
   constexpr uint32_t T = 1u << TLOG;
   __shared__ float s_box[6][T];
   SegDesc sd;
@@ -77,68 +268,18 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__rest
   }
   const uint32_t n = sd.n, N1 = n - 1;
   bvht_BVHNode *__restrict__ nodes = nodesAll + sd.nodeBase;
-  const bvht_vec3 *__restrict__ vert = vertAll + sd.vertBase;
   const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
   const uint64_t u0 = (uint64_t)group << TLOG;
   if (u0 >= U) return;  // block-uniform: a group past the last unit owns nothing
   const uint32_t cnt = (uint32_t)min((uint64_t)T, U - u0);
 
-  if (LEVEL0) {
-    // Every sorted index, face and vertex gather of the thread's leaves is
-    // issued before any is used: one chain of dependent loads per thread
-    // instead of one per leaf.
-    constexpr int R = (T + kBuildBlock - 1) / kBuildBlock;
-    uint32_t f[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const uint32_t k = threadIdx.x + i * kBuildBlock;
-      f[i] = k < cnt ? sorted[sd.faceBase + (uint32_t)(u0 + k)] - sd.faceBase : 0u;
-      if (f[i] >= n) f[i] = 0;  // never read outside the mesh, even from a corrupt permutation
-    }
-    bvht_Triangle t[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) t[i] = faces[sd.faceBase + f[i]];
-    bvht_vec3 va[R], vb[R], vc[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      va[i] = vert[t[i].v1];
-      vb[i] = vert[t[i].v2];
-      vc[i] = vert[t[i].v3];
-    }
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const uint32_t k = threadIdx.x + i * kBuildBlock;
-      if (k >= cnt) continue;
-      const uint32_t p = (uint32_t)(u0 + k);
-      const bvht_AABB b = triangleBox(va[i], vb[i], vc[i]);
-      uint32_t parent, isLeft;
-      parentOf(p, p, N1, parent, isLeft);
-      // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
-      storeNode(nodes + N1 + p, b, parent, f[i], 1u, 1u, isLeft, 255u, surfaceArea(b));
-      s_box[0][k] = b.minMax[0].x;
-      s_box[1][k] = b.minMax[0].y;
-      s_box[2][k] = b.minMax[0].z;
-      s_box[3][k] = b.minMax[1].x;
-      s_box[4][k] = b.minMax[1].y;
-      s_box[5][k] = b.minMax[1].z;
-    }
-  }
-  for (uint32_t k = threadIdx.x; !LEVEL0 && k < cnt; k += kBuildBlock) {
+  for (uint32_t k = threadIdx.x; k < cnt; k += kBuildBlock) {
     const uint64_t u = u0 + k;
-    bvht_AABB b;
-    {
-      const uint32_t l = (uint32_t)(u << unitLog);
-      const uint32_t r = (uint32_t)min((u + 1) << unitLog, (uint64_t)n) - 1;
-      const float *src = reinterpret_cast<const float *>(nodes + rangeNodeIndex(l, r, N1));
-      b.minMax[0] = {src[0], src[1], src[2]};
-      b.minMax[1] = {src[3], src[4], src[5]};
-    }
-    s_box[0][k] = b.minMax[0].x;
-    s_box[1][k] = b.minMax[0].y;
-    s_box[2][k] = b.minMax[0].z;
-    s_box[3][k] = b.minMax[1].x;
-    s_box[4][k] = b.minMax[1].y;
-    s_box[5][k] = b.minMax[1].z;
+    const uint32_t l = (uint32_t)(u << unitLog);
+    const uint32_t r = (uint32_t)min((u + 1) << unitLog, (uint64_t)n) - 1;
+    const float *src = reinterpret_cast<const float *>(nodes + rangeNodeIndex(l, r, N1));
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s_box[c][k] = src[c];
   }
   __syncthreads();
 
@@ -160,25 +301,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel(bvht_BVHNode *__rest
 #pragma unroll
       for (int c = 0; c < 6; ++c) res[q][c] = s_box[c][c0];
       if (c1 >= nPrev) continue;  // right half empty: same range as the left half
-      bvht_AABB b;
-      b.minMax[0] = {mergeMin(res[q][0], s_box[0][c1]), mergeMin(res[q][1], s_box[1][c1]),
-                     mergeMin(res[q][2], s_box[2][c1])};
-      b.minMax[1] = {mergeMax(res[q][3], s_box[3][c1]), mergeMax(res[q][4], s_box[4][c1]),
-                     mergeMax(res[q][5], s_box[5][c1])};
-      res[q][0] = b.minMax[0].x;
-      res[q][1] = b.minMax[0].y;
-      res[q][2] = b.minMax[0].z;
-      res[q][3] = b.minMax[1].x;
-      res[q][4] = b.minMax[1].y;
-      res[q][5] = b.minMax[1].z;
-      const uint32_t k0 = a << j, km = k0 + half;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) res[q][c] = mergeMin(res[q][c], s_box[c][c1]);
+#pragma unroll
+      for (int c = 3; c < 6; ++c) res[q][c] = mergeMax(res[q][c], s_box[c][c1]);
+      const uint32_t k0 = a << j;
       const uint32_t l = (uint32_t)((u0 + k0) << unitLog);
-      const uint32_t m = (uint32_t)((u0 + km) << unitLog);
       const uint32_t r = (uint32_t)min((u0 + k0 + (1u << j)) << unitLog, (uint64_t)n) - 1;
-      uint32_t parent, isLeft;
-      parentOf(l, r, N1, parent, isLeft);
-      storeNode(nodes + rangeNodeIndex(l, r, N1), b, parent, rangeNodeIndex(l, m - 1, N1), rangeNodeIndex(m, r, N1), 0u,
-                isLeft, 0u, surfaceArea(b));
+      storeInner(nodes, l, r, N1, res[q]);
     }
     __syncthreads();
 #pragma unroll
