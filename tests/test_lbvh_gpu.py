@@ -61,6 +61,22 @@ def test_fused_build_bit_exact(bvh, orc, cases, name, bits, digits):
     assert nodes_equal(got, expect)
 
 
+@pytest.mark.parametrize("tile", [8, 9, 11])
+@pytest.mark.parametrize("name", ["soup2", "soup17", "soup1000", "soup1025", "soup4097", "soup100003", "flat_dupes"])
+def test_tile_sizes_bit_exact(bvh, orc, cases, name, tile):
+    # tiles of 2^8..2^11 sorted positions, each cut at n: the inner nodes of
+    # a tile come from the lanes of their indices and from the upper levels
+    v, f = cases[name]
+    print(f"case {name} n={len(f)} tile=2^{tile}", flush=True)
+    keys, sf, expect, scene = orc.lbvh(v, f, 30)
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, 30, tile_log2=tile)
+    tree = b.allocate_bvh()
+    b.build(tree)
+    assert np.array_equal(b.sorted_faces(), sf)
+    assert nodes_equal(tree.download(), expect)
+
+
 @pytest.mark.parametrize("bits", [30, 63])
 @pytest.mark.parametrize("name", ["soup3", "soup1025", "heightfield65536", "spheres_small"])
 def test_staged_build_bit_exact(bvh, orc, cases, name, bits):
