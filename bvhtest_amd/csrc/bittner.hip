@@ -637,8 +637,8 @@ __global__ __launch_bounds__(256) void k_refitCount(const uint32_t *__restrict__
   uint32_t node = starts[i];
   if (node >= N) return;
   for (uint32_t steps = 0; steps < N; ++steps) {
+    const uint32_t p = loadRelaxedAgent(&nodes[node].parent);  // issued beside the count: one round trip per level
     if (atomicAdd(cnt + node, 1u) != 0) return;
-    const uint32_t p = nodes[node].parent;
     if (p == node) return;
     node = p;
   }
@@ -662,12 +662,17 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
     node = p;
   }
   for (uint32_t steps = 0; steps < N; ++steps) {
+    // The topology does not change during the refit, so its load goes out
+    // beside the count (the chain of last arrivals near the root is serial:
+    // each level costs the count, the child boxes and the release).  Relaxed
+    // atomic loads keep the compiler from sinking them below the count.
+    const uint32_t p = loadRelaxedAgent(&nodes[node].parent), l = loadRelaxedAgent(&nodes[node].left),
+                   r = loadRelaxedAgent(&nodes[node].right);
     // release of the boxes written on the previous trip: fence after the
     // stores, then a relaxed count (see fenceReleaseAgent)
     const uint32_t left = __hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (left != 1) return;
     fenceAcquireAgent();
-    const uint32_t l = nodes[node].left, r = nodes[node].right;
     bvht_AABB b = nodes[l].bbox;
     mergeInto(b, nodes[r].bbox);
     nodes[node].bbox = b;
@@ -678,7 +683,6 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
       subSA[node] = subSA[l] + subSA[r] + sa;
     }
     fenceReleaseAgent();
-    const uint32_t p = nodes[node].parent;
     if (p == node) return;
     node = p;
   }
