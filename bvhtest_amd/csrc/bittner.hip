@@ -674,13 +674,24 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
     if (left != 1) return;
     fenceAcquireAgent();
     bvht_AABB b = nodes[l].bbox;
-    mergeInto(b, nodes[r].bbox);
+    const bvht_AABB br = nodes[r].bbox;
+    // the children's subtree features load with their boxes, ahead of the
+    // node's stores (which may alias them), not one round trip later
+    uint32_t cl = 0, cr = 0;
+    float sl = 0.0f, sr = 0.0f;
+    if (leafCnt) {
+      cl = leafCnt[l];
+      cr = leafCnt[r];
+      sl = subSA[l];
+      sr = subSA[r];
+    }
+    mergeInto(b, br);
     nodes[node].bbox = b;
     const float sa = surfaceArea(b);
     nodes[node].surfaceArea = sa;
     if (leafCnt) {
-      leafCnt[node] = leafCnt[l] + leafCnt[r];
-      subSA[node] = subSA[l] + subSA[r] + sa;
+      leafCnt[node] = cl + cr;
+      subSA[node] = sl + sr + sa;
     }
     fenceReleaseAgent();
     if (p == node) return;
