@@ -195,6 +195,7 @@ __global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *__restrict__
   __shared__ uint32_t s_item;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t num = min(ctrl[0], listCap);
+  if (num == 0) return;  // the usual case: no long group, no work-queue atomic
   const uint64_t lt = __lanemask_lt();
   while (true) {
     if (tid == 0) s_item = atomicAdd(ctrl + 1, 1u);
