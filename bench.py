@@ -170,14 +170,14 @@ def run_bench(args):
                 pass_idx += 1
             per.setdefault(key, []).append(buf[i])
     L.bvht_LBVH_setTiming(False)
-    passes = sum(1 for k in per if k.startswith("k_onesweep"))
+    passes = 5 if bits == 63 else 4  # onesweep digit passes (63-bit keys: bits 24-62)
     stages = {k: float(np.mean(x)) for k, x in per.items()}
     stages["sort_total"] = sum(x for k, x in stages.items() if k.startswith(SORT_STAGES))
     nv = len(v)
     tile_log2 = builder.mem.opts.tileLog2
 
     def nbytes(name):
-        if name.startswith("k_onesweep"):
+        if name.startswith("k_onesweep["):
             p = int(name[len("k_onesweep["):-1])
             return kernel_bytes("k_onesweep", n, nv, bits, tile_log2, p, passes)
         return kernel_bytes(name, n, nv, bits, tile_log2)
@@ -380,6 +380,25 @@ def learned_bench(bt, seed=200, plain_steps=100):
     return r
 
 
+def host_cpu():
+    """Model name (as lscpu prints it, from /proc/cpuinfo) and the number of
+    logical CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 0
+    return model, avail
+
+
 def cpu_baseline(v, f, bits, nodes, seconds=12.0):
     """Oracle OpenMP LBVH timed on this host: repeated full builds of the same
     scene for about `seconds` with every host thread, then about a third of
@@ -415,7 +434,9 @@ def cpu_baseline(v, f, bits, nodes, seconds=12.0):
     bn_s = time.perf_counter() - t0
     print(f"# cpu baseline: trace {len(sample_rays) / trace_s / 1e6:.2f} Mrays/s, sequential Bittner13 {bn_s:.2f} s/step",
           file=sys.stderr, flush=True)
+    model, avail = host_cpu()
     return {"value": round(mtri, 3), "unit": "Mtri/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_logical_cpus": avail,
             "sample": f"{reps} full builds of the same {len(f)}-triangle scene ({bits}-bit keys, encode to refit), "
                       f"oracle OpenMP LBVH, {secs:.2f}s of build time",
             "single_thread": {"value": round(len(f) * reps1 / secs1 / 1e6, 3), "unit": "Mtri/s",

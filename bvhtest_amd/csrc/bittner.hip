@@ -41,7 +41,7 @@ constexpr int kNNode = 10;  // BVHPatch.hpp:40
 constexpr uint32_t P_GRAND_PARENT = 0, P_1ST_ROOT = 1, P_1ST_BEST = 2, P_NODE = 3, P_1ST_INSERT = 4, P_2ND_ROOT = 5,
                    P_2ND_BEST = 6, P_2ND_INSERT = 7, P_SIBLING = 8, P_PARENT = 9;  // BVHPatch.hpp:44-53
 constexpr int kStackCap = 1024;  // per-search frontier (findNode1 heap: 4096)
-constexpr int kStackCapAlt = 16; // altFindNode: bounded frontier (findNode2 keeps 16 slots)
+constexpr uint32_t kAltSlots = 16;  // findNode2's array (CUDA_ALT_QUEUE_SIZE, Bittner13CUDA.hpp:28)
 
 // Stats words (device)
 enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_UNRESOLVED, ST_RUNAWAY, ST_WORDS = 8 };
@@ -358,6 +358,71 @@ __device__ uint32_t waveSearch(const Patch &P, const bvht_BVHNode *__restrict__ 
   return bestNode;
 }
 
+// ---- findNode2 (altFindNode), Bittner13CUDA.cu:194-264 ---------------------
+//
+// The reference's approximate search, exactly: a best-first search over a
+// fixed array of 16 (node, induced cost) slots, held here by lanes 0..15 of
+// the wave (one slot per lane).  Each iteration pops the slot of minimum
+// cost; an expanded node's left child takes that slot and its right child
+// the slot of maximum cost found by the previous scan (an empty slot holds
+// HUGE_VALF, so it is the maximum while one is free; once the array is full
+// the costliest entry is overwritten).  Both scans keep the first slot of
+// their extreme (the reference's strict < and > loops); the maximum scan
+// starts from 0 and keeps the previous slot when no cost is above 0.  NaN
+// costs never compare, so they are mapped out of both scans.  One entry is
+// popped per iteration and a tree pushes each node at most once, so a
+// search of a tree ends within `maxIters` = node count iterations; a longer
+// one (a patched view that is not a tree) returns NONE.
+__device__ uint32_t altSearch(const Patch &P, const bvht_BVHNode *__restrict__ nodes, uint32_t x, uint32_t root,
+                              uint32_t maxIters, float &bestCost) {
+  const uint32_t lane = threadIdx.x;
+  const bool slot = lane < kAltSlots;
+  float xb[6];
+  globalBox(nodes, x, xb);  // getOrig(_n)->bbox
+  const float sa = nodes[x].surfaceArea;
+  uint32_t slotNode = root;
+  float slotCost = lane == 0 ? 0.0f : HUGE_VALF;
+  uint32_t minIdx = 0, maxIdx = 1;
+  float best = HUGE_VALF, lMin = 0.0f;
+  uint32_t bestNode = 0;
+  for (uint32_t it = 0; lMin < HUGE_VALF; ++it) {  // lMin, minIdx, maxIdx are wave-uniform
+    if (it >= maxIters) return NONE;
+    const uint32_t cur = (uint32_t)__shfl((int)slotNode, (int)minIdx, 64);
+    const float curCost = __shfl(slotCost, (int)minIdx, 64);
+    if (lane == minIdx) slotCost = HUGE_VALF;
+    if ((curCost + sa) >= best) break;
+    const NodeView nv = patchGetSubset(P, nodes, cur);
+    float b[6];
+    patchBox(P, nodes, cur, b);
+    float newInd = -1.0f * sa6(b);
+    merge6(b, xb);
+    const float total = curCost + sa6(b);
+    if (total < best) {
+      best = total;
+      bestNode = cur;
+    }
+    newInd += total;
+    if ((newInd + sa) < best && nv.isLeaf == 0) {
+      if (lane == minIdx) {
+        slotNode = nv.left;
+        slotCost = newInd;
+      }
+      if (lane == maxIdx) {  // written second: takes the slot if both are one
+        slotNode = nv.right;
+        slotCost = newInd;
+      }
+    }
+    const bool num = slot && !__builtin_isnan(slotCost);
+    const float cMin = num ? slotCost : HUGE_VALF, cMax = num ? slotCost : 0.0f;
+    lMin = waveMin(cMin);
+    const float lMax = waveMax(cMax);
+    if (lMin < HUGE_VALF) minIdx = (uint32_t)__builtin_ctzll(__ballot(num && cMin == lMin));
+    if (lMax > 0.0f) maxIdx = (uint32_t)__builtin_ctzll(__ballot(num && cMax == lMax));
+  }
+  bestCost = best;
+  return bestNode;
+}
+
 // Destination of SAH-gain records (feat == nullptr: not recording).
 struct RecordSink {
   const float *feat;  // features by inner-list position
@@ -386,7 +451,7 @@ template <bool PROBE>
 __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ nodes, uint32_t root, uint32_t N,
                                                const uint32_t *__restrict__ order, const uint32_t *__restrict__ inner,
                                                uint32_t numInner, uint32_t chunk, uint32_t numChunks, uint32_t K,
-                                               uint32_t offsetAccess, int cap, PatchRec *__restrict__ recs,
+                                               uint32_t offsetAccess, uint32_t alt, PatchRec *__restrict__ recs,
                                                uint64_t *locks, uint32_t *__restrict__ stats, RecordSink sink) {
   __shared__ Patch P;
   __shared__ uint32_t s_node[kStackCap];
@@ -408,7 +473,9 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
   }
   __syncthreads();
   if (s_ok) {
-    const uint32_t b1 = waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW, c1);
+    const uint32_t b1 = alt ? altSearch(P, nodes, s_ins[0], root, N, c1)
+                            : waveSearch(P, nodes, s_ins[0], root, s_node, s_cost, &s_top, kStackCap, N,
+                                         stats + ST_OVERFLOW, c1);
     if (lane == 0) {
       if (b1 == root || b1 == NONE) {
         if (b1 == NONE) atomicAdd(stats + ST_RUNAWAY, 1u);
@@ -421,7 +488,9 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
     __syncthreads();
   }
   if (s_ok) {
-    const uint32_t b2 = waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, cap, N, stats + ST_OVERFLOW, c2);
+    const uint32_t b2 = alt ? altSearch(P, nodes, s_ins[1], root, N, c2)
+                            : waveSearch(P, nodes, s_ins[1], root, s_node, s_cost, &s_top, kStackCap, N,
+                                         stats + ST_OVERFLOW, c2);
     if (lane == 0) {
       if (b2 == root || b2 == NONE) {
         if (b2 == NONE) atomicAdd(stats + ST_RUNAWAY, 1u);
@@ -655,8 +724,11 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
   if (i >= num) return;
   uint32_t node = starts[i];
   if (node >= N) return;
-  if (nodes[node].isLeafFlag != 0) {  // a leaf start (dynamic refit) is counted once and not recomputed
-    __hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (nodes[node].isLeafFlag != 0) {
+    // a leaf start (dynamic refit) is not recomputed; k_refitCount counted
+    // every occurrence of it and walked on once, so only the last
+    // occurrence walks on here (a list may name a leaf more than once)
+    if (__hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1) return;
     const uint32_t p = nodes[node].parent;
     if (p == node || p >= N) return;
     node = p;
@@ -701,8 +773,10 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
 
 // kCalcCost -- Bittner13CUDA.cu:480-500, written as an ascending sort key
 // (~bits of the non-negative cost: descending cost, ties by input order).
+// keys2 (optional) receives a second copy that the sort leaves untouched.
 __global__ __launch_bounds__(256) void k_cost(const bvht_BVHNode *__restrict__ nodes, const uint32_t *__restrict__ ids,
-                                              uint32_t num, uint32_t *__restrict__ keys, uint32_t *__restrict__ hist) {
+                                              uint32_t num, uint32_t *__restrict__ keys, uint32_t *__restrict__ keys2,
+                                              uint32_t *__restrict__ hist) {
   __shared__ uint32_t s_hist[4][kSortRadix];
   for (int p = 0; p < 4; ++p) s_hist[p][threadIdx.x] = 0;
   __syncthreads();
@@ -715,6 +789,7 @@ __global__ __launch_bounds__(256) void k_cost(const bvht_BVHNode *__restrict__ n
     }
     const uint32_t key = ~__float_as_uint(cost);
     keys[i] = key;
+    if (keys2) keys2[i] = key;
 #pragma unroll
     for (int p = 0; p < 4; ++p) atomicAdd(&s_hist[p][(key >> (8 * p)) & 255u], 1u);
   }
@@ -725,10 +800,86 @@ __global__ __launch_bounds__(256) void k_cost(const bvht_BVHNode *__restrict__ n
   }
 }
 
+// Ordered-selection tiles: 16 consecutive positions per thread.
+constexpr uint32_t kSelBlock = 256, kSelItems = 16, kSelTile = kSelBlock * kSelItems;
+
+// ---- altSort selection (bn13_selectNodes with altSort, :682-699) ----------
+//
+// The costs sorted descending give the threshold t = sorted[used] (used =
+// numChunks * chunkSize); every inner node with cost >= t is flagged
+// (kGenerateFlags, :93-97) and the flagged inner-list positions are
+// compacted in order (DeviceSelect::Flagged).  When `used` reaches the list
+// length every node is flagged.  Costs are compared as floats, decoded from
+// the ~bits sort keys.  Three launches: per-tile counts, one workgroup scans
+// the tile counts (and fills the entries a step would take beyond the
+// selected count with NONE), per-tile ordered writes.
+constexpr uint32_t kAltTile = kSelTile;  // 4096 positions per workgroup
+
+__device__ __forceinline__ bool altFlag(uint32_t key, bool all, float thr) {
+  return all || __uint_as_float(~key) >= thr;
+}
+
+__device__ __forceinline__ void altThreshold(const uint32_t *sortedKeys, uint32_t num, uint32_t used, bool &all,
+                                             float &thr) {
+  all = used >= num;
+  thr = all ? 0.0f : __uint_as_float(~sortedKeys[used]);
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_altCount(const uint32_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ sortedKeys, uint32_t num,
+                                                        uint32_t used, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_scan[kSelBlock / 64];
+  bool all;
+  float thr;
+  altThreshold(sortedKeys, num, used, all, thr);
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kAltTile + threadIdx.x * kSelItems;
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < num && altFlag(keys[base + j], all, thr)) c++;
+  const uint32_t ex = blockExclusiveScan<kSelBlock>(c, s_scan);
+  if (threadIdx.x == kSelBlock - 1) counts[blockIdx.x] = ex + c;
+}
+
+constexpr int kAltScanBlock = 1024;
+__global__ __launch_bounds__(kAltScanBlock) void k_altScan(uint32_t *__restrict__ counts, uint32_t numTiles,
+                                                           uint32_t *__restrict__ order, uint32_t used) {
+  __shared__ uint32_t s_scan[kAltScanBlock / 64];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < numTiles; b += kAltScanBlock) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t v = i < numTiles ? counts[i] : 0u;
+    const uint32_t ex = blockExclusiveScan<kAltScanBlock>(v, s_scan);
+    const uint32_t carry = s_carry;
+    if (i < numTiles) counts[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == kAltScanBlock - 1) s_carry = carry + ex + v;
+    __syncthreads();
+  }
+  for (uint32_t i = s_carry + threadIdx.x; i < used; i += kAltScanBlock) order[i] = NONE;
+}
+
+__global__ __launch_bounds__(kSelBlock) void k_altWrite(const uint32_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ sortedKeys, uint32_t num,
+                                                        uint32_t used, const uint32_t *__restrict__ offsets,
+                                                        uint32_t *__restrict__ order) {
+  __shared__ uint32_t s_scan[kSelBlock / 64];
+  bool all;
+  float thr;
+  altThreshold(sortedKeys, num, used, all, thr);
+  uint32_t c = 0;
+  const uint32_t base = blockIdx.x * kAltTile + threadIdx.x * kSelItems;
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < num && altFlag(keys[base + j], all, thr)) c++;
+  uint32_t pos = offsets[blockIdx.x] + blockExclusiveScan<kSelBlock>(c, s_scan);
+  for (uint32_t j = 0; j < kSelItems; ++j)
+    if (base + j < num && altFlag(keys[base + j], all, thr)) order[pos++] = base + j;
+}
+
 // Order-preserving compaction of node ids by leaf flag (initData's two
 // DeviceSelect::If calls, Bittner13CUDA.cu:962-978): per-block counts, a
 // host-side scan of the (few) block counts, then an ordered write.
-constexpr uint32_t kSelBlock = 256, kSelItems = 16, kSelTile = kSelBlock * kSelItems;
 
 __global__ __launch_bounds__(kSelBlock) void k_selCount(const bvht_BVHNode *__restrict__ nodes, uint32_t N,
                                                         uint32_t *__restrict__ counts) {
@@ -822,6 +973,8 @@ struct Internal {
   uint32_t *inner = nullptr, *leaves = nullptr;
   uint32_t *keys[2] = {nullptr, nullptr};
   uint32_t *vals[2] = {nullptr, nullptr};
+  uint32_t *keysAlt = nullptr;    // altSort: the keys the sort consumes (keys[0] keeps the unsorted costs)
+  uint32_t *altCounts = nullptr;  // altSort: per-tile counts / offsets
   uint32_t *sortScratch = nullptr;
   SortPlan plan;
   uint32_t *sorted = nullptr;  // -> vals[result]
@@ -979,6 +1132,8 @@ void freeMemory(bvht_GPUWorkingMemory *data) {
     devFree(in->leaves);
     devFree(in->keys[0]);
     devFree(in->keys[1]);
+    devFree(in->keysAlt);
+    devFree(in->altCounts);
     devFree(in->vals[0]);
     devFree(in->vals[1]);
     devFree(in->sortScratch);
@@ -1146,6 +1301,22 @@ void bn13_selectNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bv
     return;
   }
   if ((cfg.priority == 1 || in->recMax) && !features(in, bvh)) return;
+  const bool alt = cfg.altSort != 0;
+  if (alt && cfg.priority == 1) {
+    setError("bn13_selectNodes: altSort selects by the heuristic priority only (priority 0)");
+    return;
+  }
+  const uint64_t used = (uint64_t)cfg.numChunks * cfg.chunkSize;
+  if (alt && used > in->numInner) {
+    setError("bn13_selectNodes: altSort: %u chunks x %u exceed %u inner nodes", cfg.numChunks, cfg.chunkSize,
+             in->numInner);
+    return;
+  }
+  const uint32_t altTiles = divUp(in->numInner, kAltTile);
+  if (alt && !in->keysAlt &&
+      (!devAlloc(&in->keysAlt, in->numInner, "bn13 altSort keys") ||
+       !devAlloc(&in->altCounts, altTiles, "bn13 altSort counts")))
+    return;
   uint32_t *scratch = in->sortScratch;
   if (!sortReset(in->plan, scratch)) return;
   if (cfg.priority == 1) {
@@ -1153,8 +1324,30 @@ void bn13_selectNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bv
   } else {
     uint32_t blocks = divUp(in->numInner, 256 * 8);
     if (blocks > (uint32_t)numCUs() * 8) blocks = numCUs() * 8;
-    k_cost<<<blocks, 256, 0, stream()>>>(bvh->nodes, in->inner, in->numInner, in->keys[0], sortHist(in->plan, scratch));
+    k_cost<<<blocks, 256, 0, stream()>>>(bvh->nodes, in->inner, in->numInner, alt ? in->keysAlt : in->keys[0],
+                                         alt ? in->keys[0] : nullptr, sortHist(in->plan, scratch));
     if (!checkLaunch("k_cost")) return;
+  }
+  if (alt) {
+    // SortKeysDescending (:682-686): keysAlt and keys[1] ping-pong, keys[0]
+    // keeps the unsorted costs for the flags
+    void *keys[2] = {in->keysAlt, in->keys[1]};
+    uint32_t *vals[2] = {in->vals[0], in->vals[1]};
+    const int r = sortRun(in->plan, keys, vals, true, true, scratch);
+    if (r < 0) return;
+    const uint32_t *sortedKeys = reinterpret_cast<const uint32_t *>(keys[r]);
+    uint32_t *order = in->vals[0];
+    k_altCount<<<altTiles, kSelBlock, 0, stream()>>>(in->keys[0], sortedKeys, in->numInner, (uint32_t)used,
+                                                     in->altCounts);
+    if (!checkLaunch("k_altCount")) return;
+    k_altScan<<<1, kAltScanBlock, 0, stream()>>>(in->altCounts, altTiles, order, (uint32_t)used);
+    if (!checkLaunch("k_altScan")) return;
+    k_altWrite<<<altTiles, kSelBlock, 0, stream()>>>(in->keys[0], sortedKeys, in->numInner, (uint32_t)used,
+                                                     in->altCounts, order);
+    if (!checkLaunch("k_altWrite")) return;
+    in->sorted = order;
+    in->selected = true;
+    return;
   }
   void *keys[2] = {in->keys[0], in->keys[1]};
   uint32_t *vals[2] = {in->vals[0], in->vals[1]};
@@ -1178,10 +1371,9 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
              K, in->K, in->numInner);
     return;
   }
-  const int cap = cfg.altFindNode ? kStackCapAlt : kStackCap;
   k_search<false><<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, in->sorted, in->inner, in->numInner, chunk,
-                                          cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u, cap, in->recs, in->locks,
-                                          in->stats, sinkOf(in));
+                                          cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u, cfg.altFindNode ? 1u : 0u,
+                                          in->recs, in->locks, in->stats, sinkOf(in));
   if (!checkLaunch("k_search")) return;
   k_commitFirst<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats,
                                                      in->pending[0], in->numPending);
@@ -1334,9 +1526,9 @@ bool bvht_bn13_probe(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh
     return false;
   }
   if (!features(in, bvh)) return false;
-  const int cap = cfg.altFindNode ? kStackCapAlt : kStackCap;
   k_search<true><<<num, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, devPositions, in->inner, in->numInner, 0, 1,
-                                          num, 0u, cap, in->recs, in->locks, in->stats, sinkOf(in));
+                                          num, 0u, cfg.altFindNode ? 1u : 0u, in->recs, in->locks, in->stats,
+                                          sinkOf(in));
   return checkLaunch("k_search (probe)");
 }
 

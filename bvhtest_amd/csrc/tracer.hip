@@ -526,9 +526,9 @@ __global__ __launch_bounds__(64) void k_traceRaysDyn(const bvht_Ray *__restrict_
   }
 }
 
-// kRefitLeafBVHTris (CUDAKernels.cu:486-520): leaf boxes from the current
-// vertices (triangle box -+ FLT_EPSILON).  The leaf's SA is recomputed too
-// (the reference leaves it stale).
+// kRefitLeafBVHTris (CUDAKernels.cu:492-525): leaf boxes from the current
+// vertices (triangle box -+ FLT_EPSILON); like the reference, the leaf's
+// surfaceArea is not rewritten.
 __global__ __launch_bounds__(256) void k_refitLeafTris(const uint32_t *__restrict__ ids, uint32_t num,
                                                        bvht_BVHNode *__restrict__ nodes,
                                                        const bvht_Triangle *__restrict__ faces,
@@ -539,9 +539,7 @@ __global__ __launch_bounds__(256) void k_refitLeafTris(const uint32_t *__restric
   const uint32_t f = nodes[id].left;
   if (f >= numFaces) return;
   const bvht_Triangle t = faces[f];
-  const bvht_AABB b = triangleBox(vert[t.v1], vert[t.v2], vert[t.v3]);
-  nodes[id].bbox = b;
-  nodes[id].surfaceArea = surfaceArea(b);
+  nodes[id].bbox = triangleBox(vert[t.v1], vert[t.v2], vert[t.v3]);  // exact min/max: the reference's box
 }
 
 // Counter-based hash RNG (splitmix64 finaliser) -> uniform floats in [0, 1).

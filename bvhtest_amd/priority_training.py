@@ -100,12 +100,13 @@ def _timed_step(opt, tree):
     return time.perf_counter() - t0
 
 
-def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=32, max_factor=2.0):
+def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=32, max_factor=2.0, inspect=None):
     """Plain: `plain_steps` steps -> SAH S_p in time T_p.  Learned: steps until
     SAH <= 1.005 S_p (at most max_factor * plain_steps) -> T_l.  Step times
     include selection (features + inference for learned) and are measured
     with a device sync per step; the SAH evaluations that check each step
-    are not timed."""
+    are not timed.  inspect(tree, mesh, v, f), when given, sees the learned
+    optimizer's final tree before it is freed."""
     v, f = scene(seed, spheres, soup)
     mesh = DeviceMesh(v, f)
     res = {"seed": seed, "triangles": int(len(f))}
@@ -143,6 +144,8 @@ def evaluate(model, seed, spheres, soup, plain_steps, batch_percent=1.0, chunks=
     res.update(learned_steps=steps, learned_s=tl, learned_sah=sah, fallback_steps=fallbacks,
                reached=bool(sah <= target), time_ratio=(tl / tp) if sah <= target else None,
                learned_tree_valid=total == 0)
+    if inspect:
+        inspect(tree, mesh, v, f)
     opt.free()
     tree.free()
     return res

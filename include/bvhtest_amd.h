@@ -312,7 +312,28 @@ int bvht_LBVH_autotuneBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *
 /* Bittner13 GPU optimizer -- lib/builder/Bittner13CUDA.hpp:56-83           */
 /* ------------------------------------------------------------------------ */
 
-/* AlgoCFG -- lib/builder/Bittner13CUDA.hpp:56-66 (+ extension fields). */
+/* AlgoCFG -- lib/builder/Bittner13CUDA.hpp:56-66 (+ extension fields).
+ *   offsetAccess  chunk k takes sorted entry k * numChunks + c instead of
+ *                 c * chunkSize + k (Bittner13CUDA.cu:516).
+ *   altFindNode   findNode2 (Bittner13CUDA.cu:194-264): the 16-slot
+ *                 approximate search, reproduced exactly; otherwise the
+ *                 exact branch-and-bound search of findNode1.
+ *   altSort       threshold selection (Bittner13CUDA.cu:682-699): the nodes
+ *                 whose priority is at least the sorted priority at index
+ *                 numChunks * chunkSize, in inner-list order; otherwise the
+ *                 stable descending sort.  Heuristic priority only.
+ *   sort          must be true (the reference's alternative is disabled:
+ *                 Bittner13CUDA.cu:701-722); false is an error.
+ *   altFixTree    the reference refits the touched paths (true) or the whole
+ *                 tree (false) after each chunk; boxes are exact min/max,
+ *                 so both give the same bytes and the touched-path refit
+ *                 always runs here.
+ *   localPatchCPY the reference keeps a patch in registers until it is
+ *                 complete (true) or builds it in global memory, where a
+ *                 failed patch is later applied half done (false, a defect);
+ *                 this library always keeps it local.
+ *   blockSize     the reference's CUDA thread-block size; the kernels here
+ *                 pick their own launch shapes.  Results do not depend on it. */
 typedef struct bvht_AlgoCFG {
   uint32_t numChunks;
   uint32_t chunkSize;
@@ -362,15 +383,17 @@ void bn13_doAlgorithmStep(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
  * into out[]; their number into *numOut (host). */
 void selectDynamicLeafNodes(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, uint32_t *out,
                             uint32_t *numOut, uint32_t beginDynamicTriangle);
-/* refitDynamicTris -- lib/tracer/CUDAKernels.cu:486-530: leaf boxes of
- * nodesToUpdate[0..numNodes) from the current vertices (the leaf SA is
- * recomputed as well; the reference leaves it stale). */
+/* refitDynamicTris -- lib/tracer/CUDAKernels.cu:492-530: leaf boxes of
+ * nodesToUpdate[0..numNodes) from the current vertices, widened by
+ * FLT_EPSILON; as in the reference, the leaves' surfaceArea is not
+ * rewritten. */
 void refitDynamicTris(bvht_CUDAMemoryBVHPointer *bvh, bvht_MeshRaw mesh, uint32_t *nodesToUpdate, uint32_t numNodes,
                       uint32_t blockSize);
 /* fixTree3 from explicit start nodes (extension): the reference's viewer
  * points GPUWorkingMemory.nodesToFix at the dynamic leaves and calls fixTree3
  * (lib/viewer/LiveTracer.cpp:284-294); with the opaque working memory here
- * the starts are passed directly.  Leaf starts are allowed. */
+ * the starts are passed directly.  Leaf starts are allowed, and a start may
+ * appear more than once. */
 bool bvht_fixTreeFrom(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh, const uint32_t *devStarts,
                       uint32_t num);
 /* CUDAcalcSAH -- lib/builder/Bittner13CUDA.cu:776-803: (sum SA - SA(root)) / SA(root) */

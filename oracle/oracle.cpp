@@ -575,6 +575,146 @@ void orc_trace(const Ray *rays, uint32_t num, const Node *nodes, uint32_t root, 
   }
 }
 
+// kTraceRayBundle -- lib/tracer/CUDAKernels.cu:240-383: the rays of an 8x8
+// pixel tile (one thread block) walk one shared node sequence.  At an inner
+// node every ray tests both child boxes against its own nearest hit; a child
+// is entered when any ray hits it; when both are, the trail bit is set and
+// the majority prefers the near child (a ray votes +1 for the left child,
+// -1 for the right, 0 without preference; a sum > 0 enters the left child,
+// otherwise the right).  The other child is entered later through the
+// trail (Afra & Szirmay-Kalos backtracking, as kTraceRay) without a second
+// box test.  At a leaf every ray tests the triangles and keeps the first one
+// at a strictly smaller distance.  intCount counts the inner nodes the
+// bundle visits (the same for every ray of a tile).  Lanes of a tile that
+// lie outside the image take no part (in the reference their vote entries
+// are left unwritten).  Shading as kTraceRay.
+void orc_trace_bundle(const Ray *rays, uint32_t w, uint32_t h, const Node *nodes, uint32_t root, const float *vert,
+                      const uint32_t *faces, const float *light, Pixel *pix, Hit *hits) {
+  const V3 *V = reinterpret_cast<const V3 *>(vert);
+  const V3 L = {light[0], light[1], light[2]};
+  const uint32_t tx = (w + 7) / 8, ty = (h + 7) / 8;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t tile = 0; tile < (int64_t)tx * ty; ++tile) {
+    const uint32_t bx = (uint32_t)(tile % tx) * 8, by = (uint32_t)(tile / tx) * 8;
+    bool valid[64];
+    uint32_t idx[64], closest[64];
+    float nearest[64];
+    for (int l = 0; l < 64; ++l) {
+      const uint32_t x = bx + (l & 7), y = by + (l >> 3);
+      valid[l] = x < w && y < h;
+      idx[l] = valid[l] ? y * w + x : 0;
+      nearest[l] = HUGE_VALF;
+      closest[l] = UINT32_MAX;
+    }
+    uint16_t intCount = 0;
+    unsigned __int128 stack = 0;
+    uint32_t node = root;
+    float dist;
+    while (true) {
+      const Node &nd = nodes[node];
+      if (nd.isLeafFlag == 0) {
+        intCount++;
+        const uint32_t lc = nd.left, rc = nd.right;
+        bool anyL = false, anyR = false;
+        int vote = 0;
+        for (int l = 0; l < 64; ++l) {
+          if (!valid[l]) continue;
+          float minL = 0.0f, minR = 0.0f;
+          const Ray &ray = rays[idx[l]];
+          const bool hl = hitBox(nodes[lc].bbox, ray, 0.01f, nearest[l] + 0.01f, minL);
+          const bool hr = hitBox(nodes[rc].bbox, ray, 0.01f, nearest[l] + 0.01f, minR);
+          anyL |= hl;
+          anyR |= hr;
+          vote += (hl && (minL < minR || !hr)) - (hr && (minR < minL || !hl));
+        }
+        if (anyL || anyR) {
+          stack <<= 1;
+          if (anyL && anyR) {
+            stack |= 1;
+            node = vote > 0 ? lc : rc;
+          } else {
+            node = anyL ? lc : rc;
+          }
+          continue;
+        }
+      } else {
+        for (int l = 0; l < 64; ++l) {
+          if (!valid[l]) continue;
+          const Ray &ray = rays[idx[l]];
+          for (uint32_t k = 0; k < nd.right; ++k) {
+            const uint32_t t = nd.left + k;
+            const uint32_t *f = faces + 3 * t;
+            if (hitTri(ray.pos, ray.dir, V[f[0]], V[f[1]], V[f[2]], dist, true) && dist < nearest[l]) {
+              nearest[l] = dist;
+              closest[l] = t;
+            }
+          }
+        }
+      }
+      bool end = false;
+      while ((stack & 1) == 0) {
+        if (stack == 0) {
+          end = true;
+          break;
+        }
+        node = nodes[node].parent;
+        stack >>= 1;
+      }
+      if (end) break;
+      const uint32_t p = nodes[node].parent;
+      node = nodes[node].isLeft == 0 ? nodes[p].left : nodes[p].right;
+      stack ^= 1;
+    }
+    for (int l = 0; l < 64; ++l) {
+      if (!valid[l]) continue;
+      Pixel res = {0, 0, intCount};
+      if (nearest[l] < HUGE_VALF) {
+        const Ray &ray = rays[idx[l]];
+        const uint32_t *f = faces + 3 * closest[l];
+        V3 nrm = normalize(cross(sub(V[f[1]], V[f[0]]), sub(V[f[2]], V[f[0]])));
+        V3 hp = add(ray.pos, mul(ray.dir, nearest[l]));
+        V3 ld = normalize(sub(L, hp));
+        float diff = 1.0f + dot(nrm, ld);
+        diff = diff > 0.0f ? diff : 0.0f;
+        res.hit = 1;
+        res.diffuse = (uint8_t)(diff * 127.0f);
+      }
+      if (pix) pix[idx[l]] = res;
+      if (hits) hits[idx[l]] = Hit{closest[l], nearest[l]};
+    }
+  }
+}
+
+// kRefitLeafBVHTris -- lib/tracer/CUDAKernels.cu:492-525: the box of each
+// listed leaf from its (first) triangle's current vertices, widened by
+// FLT_EPSILON; the node is written back with its old surfaceArea (the kernel
+// does not recompute it).
+void orc_refit_leaf_tris(Node *nodes, const float *vert, const uint32_t *faces, const uint32_t *ids, uint32_t num) {
+  const V3 *V = reinterpret_cast<const V3 *>(vert);
+  for (uint32_t i = 0; i < num; ++i) {
+    Node &n = nodes[ids[i]];
+    const uint32_t *f = faces + 3 * n.left;
+    const V3 a = V[f[0]], b = V[f[1]], c = V[f[2]];
+    Box bb = {a, b};
+    if (b.x < bb.mn.x) bb.mn.x = b.x;
+    if (b.y < bb.mn.y) bb.mn.y = b.y;
+    if (b.z < bb.mn.z) bb.mn.z = b.z;
+    if (c.x < bb.mn.x) bb.mn.x = c.x;
+    if (c.y < bb.mn.y) bb.mn.y = c.y;
+    if (c.z < bb.mn.z) bb.mn.z = c.z;
+    if (a.x > bb.mx.x) bb.mx.x = a.x;
+    if (a.y > bb.mx.y) bb.mx.y = a.y;
+    if (a.z > bb.mx.z) bb.mx.z = a.z;
+    if (c.x > bb.mx.x) bb.mx.x = c.x;
+    if (c.y > bb.mx.y) bb.mx.y = c.y;
+    if (c.z > bb.mx.z) bb.mx.z = c.z;
+    const float e = std::numeric_limits<float>::epsilon();
+    bb.mn = {bb.mn.x - e, bb.mn.y - e, bb.mn.z - e};
+    bb.mx = {bb.mx.x + e, bb.mx.y + e, bb.mx.z + e};
+    n.bbox = bb;
+  }
+}
+
 // Brute force over all faces (lib/tracer/BruteForceTracer.cpp:56-120) with
 // the GPU triangle test above, restricted to tMin < t: the independent check
 // of the traversal on scenes whose hits all lie in front of the camera.

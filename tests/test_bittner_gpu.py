@@ -2,8 +2,9 @@
 
 Two oracles (oracle/oracle_bittner.cpp, orc_bittner13_gpuref):
   * search=1 replays this library's device algorithm serially (wave search,
-    rank-ordered first-come locks).  The device tree must equal it byte for
-    byte after every chunk.
+    rank-ordered first-come locks); search=2 is the reference's findNode2
+    (altFindNode).  The device tree must equal it byte for byte after every
+    chunk, for every option path (altFindNode, altSort, offsetAccess).
   * search=0 is the reference algorithm (findNode1's heap with the
     reference's own heap routines, lib/builder/CUDAHeap.hpp, and
     kCheckConflicts serialised in thread order).  The two differ only in
@@ -72,15 +73,15 @@ def _scene(bvh, kind, seed):
     return bvh.scene_soup(4000, seed)
 
 
-@pytest.mark.parametrize("kind,seed", [("heightfield", 21), ("spheres", 22), ("spheres", 23), ("soup", 24)])
-def test_bittner_gpu_equals_replay_after_every_chunk(bvh, orc, kind, seed):
-    v, f = _scene(bvh, kind, seed)
-    steps, chunks, batch = 3, 4, 5.0
-    print(f"{kind} seed={seed} n={len(f)} steps={steps} chunks={chunks} batch={batch}", flush=True)
+def replay_check(bvh, orc, v, f, steps, chunks, batch, search, **opts):
+    """Runs the device optimizer chunk by chunk and compares the tree with the
+    oracle's serial run of the same algorithm after every chunk."""
     mesh, tree, nodes = build(bvh, orc, v, f)
-    rc, _, _, rep_stats, trees = orc.bittner_gpuref(nodes, batch, chunks, steps, search=1, chunk_trees=True)
+    rc, _, _, rep_stats, trees = orc.bittner_gpuref(nodes, batch, chunks, steps, search=search, chunk_trees=True,
+                                                    offset_access=opts.get("offsetAccess", False),
+                                                    alt_sort=opts.get("altSort", False))
     assert rc == 0 and rep_stats[4] == 0
-    opt = bvh.Bittner13GPU(maxNumStepps=steps, numChunks=chunks, batchPercent=batch)
+    opt = bvh.Bittner13GPU(maxNumStepps=steps, numChunks=chunks, batchPercent=batch, **opts)
     opt.setup(tree, len(f))
     L = bvh.lib()
     cfg = opt.cfg(opt.chunk)
@@ -100,7 +101,66 @@ def test_bittner_gpu_equals_replay_after_every_chunk(bvh, orc, kind, seed):
     st = opt.stats()
     print("stats", st, "replay", rep_stats, flush=True)
     assert st["runaway"] == 0 and st["unresolved"] == 0
+    assert (st["conflicts"], st["failed"], st["contained"]) == tuple(int(x) for x in rep_stats[:3])
     check_tree(orc, tree.download(), v, f)
+    return st
+
+
+@pytest.mark.parametrize("kind,seed", [("heightfield", 21), ("spheres", 22), ("spheres", 23), ("soup", 24)])
+def test_bittner_gpu_equals_replay_after_every_chunk(bvh, orc, kind, seed):
+    v, f = _scene(bvh, kind, seed)
+    print(f"{kind} seed={seed} n={len(f)}", flush=True)
+    replay_check(bvh, orc, v, f, 3, 4, 5.0, search=1)
+
+
+# The reference's option paths.  With altFindNode the serial run is the
+# reference's own algorithm (findNode2, Bittner13CUDA.cu:194-264, with the
+# thread-ordered lock conflicts of kCheckConflicts), so the device tree equals
+# the reference algorithm's byte for byte; altSort is the threshold selection
+# (:682-699); offsetAccess the strided chunk order (:516).
+OPTION_CASES = [
+    ("altFindNode", dict(altFindNode=True), 2),
+    ("altSort", dict(altSort=True), 1),
+    ("altSort+altFindNode", dict(altSort=True, altFindNode=True), 2),
+    ("offsetAccess", dict(offsetAccess=True), 1),
+    ("offsetAccess+altFindNode", dict(offsetAccess=True, altFindNode=True), 2),
+]
+
+
+@pytest.mark.parametrize("name,opts,search", OPTION_CASES, ids=[c[0] for c in OPTION_CASES])
+@pytest.mark.parametrize("kind,seed", [("heightfield", 31), ("spheres", 32)])
+def test_bittner_gpu_options_equal_replay(bvh, orc, name, opts, search, kind, seed):
+    v, f = _scene(bvh, kind, seed)
+    print(f"{name} {kind} seed={seed} n={len(f)}", flush=True)
+    replay_check(bvh, orc, v, f, 3, 4, 5.0, search=search, **opts)
+
+
+def test_bittner_gpu_alt_find_node_sah_against_reference(bvh, orc):
+    """altFindNode is the reference algorithm itself (findNode2 + serialised
+    conflicts): the final SAH equals the oracle's exactly, not within 0.5%."""
+    v, f = bvh.scene_heightfield(128, 64, 3, 0.2)
+    mesh, tree, nodes = build(bvh, orc, v, f)
+    opt = bvh.Bittner13GPU(maxNumStepps=10, numChunks=8, batchPercent=10.0, altFindNode=True)
+    sahs = opt.run(tree, len(f), sah_every=10)
+    rc, ref_nodes, ref_sah, _ = orc.bittner_gpuref(nodes, 10.0, 8, 10, search=2)
+    assert rc == 0
+    assert tree.download().tobytes() == ref_nodes.tobytes()
+    assert sahs[-1] == pytest.approx(ref_sah[-1], rel=1e-6)
+
+
+def test_bittner_gpu_flags_without_effect_on_results(bvh, orc):
+    """altFixTree, localPatchCPY and blockSize choose implementation details in
+    the reference (full vs touched-path refit, patch in registers vs global
+    memory, thread-block size); the resulting tree does not depend on them."""
+    v, f = bvh.scene_spheres_soup(6, 600, 41)
+    ref = None
+    for kw in (dict(), dict(altFixTree=True), dict(localPatchCPY=True), dict(CUDABlockSize=32),
+               dict(altFixTree=True, localPatchCPY=True, CUDABlockSize=256)):
+        _, tree, _ = build(bvh, orc, v, f)
+        bvh.Bittner13GPU(maxNumStepps=3, numChunks=4, batchPercent=5.0, **kw).run(tree, len(f))
+        got = tree.download().tobytes()
+        ref = got if ref is None else ref
+        assert got == ref, kw
 
 
 @pytest.mark.skip(reason="went silent for 180 s on the GPU (96 spheres + 8000 soup triangles, 2% / 16 chunks); "
@@ -114,15 +174,6 @@ def test_bittner_gpu_deterministic(bvh, orc):
     a, b = tree_a.download(), tree_b.download()
     check_tree(orc, a, v, f)
     assert a.tobytes() == b.tobytes()
-
-
-def test_bittner_gpu_offset_access_and_alt_find(bvh, orc):
-    v, f = bvh.scene_heightfield(96, 48, 5, 0.2)
-    for kw in (dict(offsetAccess=True), dict(altFindNode=True)):
-        _, tree, nodes = build(bvh, orc, v, f)
-        s = bvh.Bittner13GPU(maxNumStepps=6, numChunks=8, batchPercent=5.0, **kw).run(tree, len(f), sah_every=6)
-        check_tree(orc, tree.download(), v, f)
-        assert s[-1] < s[0]
 
 
 def test_fix_tree_full_refit_is_identity(bvh, orc):

@@ -1,0 +1,47 @@
+"""Learned removal priority on a held-out 16.9M-triangle scene.
+
+The north-star configuration of the learned priority: the shipped 12-64-64-1
+BF16 MLP (bvhtest_amd/data/priority_mlp.npy, trained on records of scenes
+with seeds 100-131) against the plain Bittner13 heuristic (Bittner13CUDA.cu:
+480-500) on a scene it never saw (seed 200: 8,192 icospheres + 2,000,000 soup
+triangles).  The plain optimizer runs 100 steps of 1% in 32 chunks to SAH S_p;
+the learned one runs until its SAH(1.2, 1) is within 0.5% of S_p.  It must get
+there in at most half the plain optimizer's GPU time, and its final tree must
+pass the oracle's Validate (lib/misc/Validate.cpp:42-263), with the oracle's
+SAH (lib/base/BVH.cpp:23-37) equal to the device's.
+"""
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.timeout(900)
+def test_learned_priority_reaches_plain_sah_in_half_the_time(bvh, orc):
+    from bvhtest_amd import priority_training as pt
+
+    model = bvh.PriorityModel.load(os.path.join(ROOT, "bvhtest_amd", "data", "priority_mlp.npy"))
+    seen = {}
+
+    def inspect(tree, mesh, v, f):
+        seen["nodes"] = tree.download()
+        seen["sah"] = bvh.calc_sah(tree, 1.2, 1.0)
+        seen["v"], seen["f"] = v, f
+
+    res = pt.evaluate(model, 200, 8192, 2000000, 100, inspect=inspect)
+    print(res, flush=True)
+    assert res["triangles"] > 16_000_000
+    assert res["reached"], res
+    assert res["time_ratio"] <= 0.5, res
+    assert res["learned_tree_valid"]
+    nodes = seen["nodes"]
+    assert orc.tree_is_valid(nodes, 0)
+    lv = nodes.copy()
+    max_level = orc.fix_levels(lv, 0)
+    total, err = orc.validate(lv, 0, max_level, seen["v"], seen["f"])
+    assert total == 0, err
+    assert orc.calc_sah(nodes, 0, 1.2, 1.0) == pytest.approx(seen["sah"], rel=1e-6)
+    assert seen["sah"] == pytest.approx(res["learned_sah"], rel=1e-12)

@@ -296,3 +296,82 @@ def test_bittner_gpuref_improves_valid_tree(orc):
     total, err = orc.validate(opt, 0, ml, v, f)
     assert total == 0, err
     assert sah[-1] < 0.9 * sah[0]
+
+
+def test_bundle_of_one_ray_is_the_single_ray_walk(orc):
+    """A tile with one valid lane votes with that ray's own preference (ties
+    of the two entry distances go right, as in kTraceRay), so the bundle
+    walk is the single-ray walk: pixel (including intCount) and hit equal."""
+    v, f = heightfield(40, 20)
+    _, _, nodes, _ = orc.lbvh(v, f)
+    rays = orc.gen_rays(24, 24, (0.5, 1.0, 1.25), (0.5, 0.1, 0.25), (0, 1, 0), 45.0)
+    pix, hits = orc.trace(rays, nodes, 0, v, f)
+    for i in range(len(rays)):
+        bp, bh = orc.trace_bundle(rays[i:i + 1], 1, 1, nodes, 0, v, f)
+        assert bp.tobytes() == pix[i:i + 1].tobytes(), i
+        assert bh.tobytes() == hits[i:i + 1].tobytes(), i
+
+
+@pytest.mark.parametrize("scene", ["heightfield", "soup"])
+def test_bundle_finds_every_rays_nearest_hit(orc, scene):
+    """The shared walk enters every child some ray's box test accepts, which
+    includes every node on the path to each ray's nearest triangle, so each
+    ray ends with its own nearest distance: t equals the single-ray walk and
+    brute force for every ray; the triangle differs only where two lie at
+    exactly that distance; intCount is one value per 8x8 tile."""
+    if scene == "heightfield":
+        v, f = heightfield(64, 32)
+        cam = ((0.5, 1.0, 1.25), (0.5, 0.1, 0.25))
+    else:
+        v, f = tri_mesh_soup(3000, seed=4, spread=0.08)
+        cam = ((0.5, 0.5, 2.0), (0.5, 0.5, 0.5))
+    _, _, nodes, _ = orc.lbvh(v, f)
+    w, h = 77, 45  # partial tiles on both borders
+    rays = orc.gen_rays(w, h, cam[0], cam[1], (0, 1, 0), 45.0)
+    pix, hits = orc.trace(rays, nodes, 0, v, f)
+    bpix, bhits = orc.trace_bundle(rays, w, h, nodes, 0, v, f)
+    bf = orc.trace_bruteforce(rays, v, f, tmin=0.0)
+    hit = hits["triId"] != 0xFFFFFFFF
+    assert hit.sum() > 0.1 * len(rays)
+    assert np.array_equal(bhits["triId"] != 0xFFFFFFFF, hit)
+    assert np.array_equal(bpix["hit"], pix["hit"])
+    assert np.array_equal(bhits["t"], hits["t"])
+    assert np.array_equal(bhits["t"][hit], bf["t"][hit])
+    diff = np.nonzero(bhits["triId"] != hits["triId"])[0]
+    for i in diff:  # a tie: the bundle's triangle is hit at the same distance
+        one = orc.trace_bruteforce(rays[i:i + 1], v, f[bhits["triId"][i]:bhits["triId"][i] + 1], tmin=0.0)
+        assert one["t"][0] == hits["t"][i]
+    ic = bpix["intCount"].reshape(h, w)
+    for ty in range(0, h, 8):
+        for tx in range(0, w, 8):
+            assert len(np.unique(ic[ty:ty + 8, tx:tx + 8])) == 1
+
+
+def test_leaf_refit_matches_triangle_boxes_and_keeps_area(orc):
+    """kRefitLeafBVHTris: the leaf box equals the triangle's box widened by
+    FLT_EPSILON (kInitTriData's box, pinned by numpy min/max) and the
+    surface area field is left as it was; a full refit afterwards equals the
+    union of the new leaf boxes."""
+    v, f = tri_mesh_soup(500, seed=3)
+    _, _, nodes, _ = orc.lbvh(v, f)
+    v2 = v + np.float32(0.25)
+    leaves = np.nonzero(nodes["isLeafFlag"])[0][::3].astype(np.uint32)
+    out = orc.refit_leaf_tris(nodes, v2, f, leaves)
+    tv = v2[f[nodes["left"][leaves]]]
+    eps = F32_EPS
+    assert np.array_equal(out["bbox"][leaves][:, :3], (tv.min(axis=1) - eps).astype(np.float32))
+    assert np.array_equal(out["bbox"][leaves][:, 3:], (tv.max(axis=1) + eps).astype(np.float32))
+    assert np.array_equal(out["surfaceArea"], nodes["surfaceArea"])
+    others = np.setdiff1d(np.arange(len(nodes)), leaves)
+    assert out[others].tobytes() == nodes[others].tobytes()
+    full = orc.fix_tree1(out)
+    assert orc.tree_is_valid(full, 0)
+    inner = np.nonzero(full["isLeafFlag"] == 0)[0]
+    for i in inner[:2000]:
+        l, r = full["left"][i], full["right"][i]
+        u = np.concatenate([np.minimum(full["bbox"][l][:3], full["bbox"][r][:3]),
+                            np.maximum(full["bbox"][l][3:], full["bbox"][r][3:])])
+        assert np.array_equal(full["bbox"][i], u)
+    assert full["bbox"][0][0] == min(full["bbox"][nodes["isLeafFlag"] != 0][:, 0])
+    # a built tree is already exact: the full refit changes nothing
+    assert orc.fix_tree1(nodes).tobytes() == nodes.tobytes()
