@@ -70,9 +70,13 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
 
 
 def profile_build(args):
-    """Counter runs (rocprofv3 --pmc): the timed LBVH builds only, then a
-    device-to-device copy of 256 MiB, the calibration kernel whose counters
-    per byte scale the build's FETCH_SIZE / WRITE_SIZE to bytes."""
+    """Counter runs (rocprofv3 --pmc): the timed LBVH builds only, then three
+    calibration kernels of known bytes whose counters per byte scale the
+    build's FETCH_SIZE / WRITE_SIZE to bytes: a device-to-device copy of
+    256 MiB (reads and writes 256 MiB), a streaming read (a sum over
+    256 MiB) and a gather (every 128-byte row of 256 MiB read once, in a
+    random order, and written contiguously).  The last two run through
+    torch, used here only as a source of kernels."""
     import bvhtest_amd as bt
 
     L = bt.lib()
@@ -89,7 +93,20 @@ def profile_build(args):
         if not L.runMemcpy(b.ptr, a.ptr, nbytes, 3):
             raise SystemExit("device copy failed")
     L.LBVH_doCUDASyc()
-    print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes}), flush=True)
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    rows = x.view(-1, 32)
+    perm = torch.randperm(rows.shape[0], device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        x.sum()
+    for _ in range(3):
+        rows.index_select(0, perm)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes, "read_bytes": nbytes,
+                      "gather_rows": int(rows.shape[0]), "gather_row_bytes": 128,
+                      "gather_index_bytes": int(perm.numel() * perm.element_size())}), flush=True)
 
 
 def profile_batch(args):

@@ -15,9 +15,14 @@ Two oracles (oracle/oracle_bittner.cpp, orc_bittner13_gpuref):
 Every tree must pass the Validate checks (lib/misc/Validate.cpp:42-263).
 """
 import ctypes
+import hashlib
+import json
+import os
 
 import numpy as np
 import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -206,4 +211,35 @@ def test_bittner_gpu_full_scene_valid(bvh, orc):
     assert stats["runaway"] == 0 and stats["unresolved"] == 0 and stats["overflows"] == 0
     assert stats["applied"] > 0
     assert all(b < a for a, b in zip(sahs, sahs[1:]))
+    check_tree(orc, tree.download(), v, f)
+
+
+def test_bittner_gpu_at_scale_against_reference_fixture(bvh, orc):
+    """Bittner13 at its own scale (Bittner13CUDA.cu:768-772 with the defaults
+    of Bittner13GPU.hpp:28-37): 100 steps of 1% of the nodes in 32 chunks on
+    the ~4.24M-triangle scene of 2,048 icospheres + 500,000 soup triangles.
+    The reference algorithm's SAH after each step (findNode1, serialised lock
+    conflicts) is the fixture tests/golden/c2_scale.json, written by
+    tests/golden/make_c2_scale.py from the oracle.  The device must start
+    from the same tree, end within 0.5% of the reference's final SAH, resolve
+    every lock and leave a valid tree."""
+    with open(os.path.join(GOLDEN, "c2_scale.json")) as fh:
+        fx = json.load(fh)
+    sc = fx["scene"]
+    v, f = bvh.scene_spheres_soup(sc["spheres"], sc["soup"], sc["seed"])
+    mesh = bvh.DeviceMesh(v, f)
+    tree = bvh.lbvh_build(mesh, 63)
+    assert hashlib.sha256(tree.download().tobytes()).hexdigest() == fx["start_nodes_sha256"]
+    run = fx["run"]
+    opt = bvh.Bittner13GPU(maxNumStepps=run["steps"], numChunks=run["num_chunks"], batchPercent=run["batch_percent"])
+    sahs = opt.run(tree, len(f), sah_every=10)
+    st = opt.stats()
+    ref = fx["sah"]
+    print("step, device SAH, reference SAH, ratio", flush=True)
+    for i, s_dev in enumerate(sahs):
+        print(10 * i, round(s_dev, 4), round(ref[10 * i], 4), round(s_dev / ref[10 * i], 5), flush=True)
+    print("stats", st, "reference", fx["stats"], flush=True)
+    assert sahs[0] == pytest.approx(ref[0], rel=1e-6)
+    assert st["runaway"] == 0 and st["unresolved"] == 0 and st["overflows"] == 0
+    assert abs(sahs[-1] / ref[-1] - 1.0) <= 0.005
     check_tree(orc, tree.download(), v, f)
