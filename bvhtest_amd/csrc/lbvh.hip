@@ -429,6 +429,24 @@ bool runTree(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMe
                                            : nullptr);
 }
 
+// The whole build on the stream: scene box, keys, sort, tree.
+bool buildAll(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh) {
+  Internal *in = internalOf(mem);
+  in->mesh = *mesh;
+  in->haveMesh = true;
+  Timing &tm = timing();
+  tm.start();
+  tm.record("begin");
+  k_sceneBox<<<in->boxBlocks, kBoxBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
+                                                     in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
+  if (!checkLaunch("k_sceneBox")) return false;
+  tm.record("k_sceneBox");
+  if (!launchMorton(mem, mesh)) return false;
+  tm.record("k_morton");
+  if (!runSort(mem)) return false;
+  return runTree(mem, mesh, bvh);
+}
+
 }  // namespace
 
 template <int TLOG>
@@ -724,21 +742,7 @@ bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvh
     setError("bvht_LBVH_build: BVH not allocated for %u faces", mesh->numFaces);
     return false;
   }
-  Internal *in = internalOf(mem);
-  in->mesh = *mesh;
-  in->haveMesh = true;
-  Timing &tm = timing();
-  tm.start();
-  tm.record("begin");
-  k_sceneBox<<<in->boxBlocks, kBoxBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
-                                                     in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
-  if (!checkLaunch("k_sceneBox")) return false;
-  tm.record("k_sceneBox");
-  if (!launchMorton(mem, mesh)) return false;
-  tm.record("k_morton");
-  if (!runSort(mem)) return false;
-  if (!runTree(mem, mesh, bvh)) return false;
-  return true;
+  return buildAll(mem, mesh, bvh);
 }
 
 uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem) {
@@ -780,3 +784,111 @@ const char *bvht_LBVH_stageName(int i) {
 }
 
 }  // extern "C"
+
+// ---- a build that allocates nothing ----------------------------------------
+//
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops): the working memory of bvht_LBVH_build lives
+// in `scratch`, a file-scope device array that every GPU the library runs on
+// holds until the process ends.  Meshes of up to kNoAllocMaxFaces faces fit
+// (30- or 63-bit keys, default options otherwise).  Every call on one GPU
+// uses that GPU's array, so calls that overlap on one GPU race; the array
+// keeps the last call's contents, and the parts that must start at zero are
+// cleared by the call itself (fill kernels, no memset nodes).
+
+constexpr uint32_t kNoAllocMaxFaces = 1u << 20;
+constexpr size_t kNoAllocBytes = 28u << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace bvht {
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// The device address of `scratch` on the current device (looked up once per
+// device with that device current; each GPU holds its own copy).
+uint8_t *scratchAddress() {
+  static uint8_t *addr[kMaxDevices] = {};
+  int dev = -1;
+  if (!check(hipGetDevice(&dev), "bvht_LBVH_buildNoAlloc device")) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    setError("bvht_LBVH_buildNoAlloc: device %d out of range", dev);
+    return nullptr;
+  }
+  if (!addr[dev]) {
+    void *p = nullptr;
+    if (!check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "bvht_LBVH_buildNoAlloc scratch address")) return nullptr;
+    addr[dev] = static_cast<uint8_t *>(p);
+  }
+  return addr[dev];
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Carves the working memory of an n-face build out of `base`; returns the
+// bytes used (the plans in `in` must be set).
+size_t carve(uint8_t *base, uint32_t n, int keyBytes, bvht_LBVH_WorkingMemory &m, Internal &in) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    uint8_t *p = base ? base + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  m.keys[0] = take((size_t)keyBytes * n);
+  m.keys[1] = take((size_t)keyBytes * n);
+  m.values[0] = reinterpret_cast<uint32_t *>(take(4ull * n));
+  m.values[1] = reinterpret_cast<uint32_t *>(take(4ull * n));
+  m.sortScratch = take(m.sortScratchBytes);
+  m.params = take(sizeof(BuildParams));
+  in.partials = reinterpret_cast<float *>(take(sizeof(float) * kBoxBlocksMax * 6));
+  return off;
+}
+
+}  // namespace
+}  // namespace bvht
+
+extern "C" bool bvht_LBVH_buildNoAlloc(const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh, uint32_t mortonBits) {
+  if (!validateMesh(mesh, "bvht_LBVH_buildNoAlloc")) return false;
+  if (mortonBits != 30 && mortonBits != 63) {
+    setError("bvht_LBVH_buildNoAlloc: mortonBits must be 30 or 63 (got %u)", mortonBits);
+    return false;
+  }
+  const uint32_t n = mesh->numFaces;
+  if (n > kNoAllocMaxFaces) {
+    setError("bvht_LBVH_buildNoAlloc: %u faces exceed the maximum of %u for the build without allocation", n,
+             kNoAllocMaxFaces);
+    return false;
+  }
+  if (!bvh || !bvh->nodes || bvh->numNodes != 2 * n - 1) {
+    setError("bvht_LBVH_buildNoAlloc: BVH not allocated for %u faces", n);
+    return false;
+  }
+  Internal in;
+  bvht_LBVH_WorkingMemory m;
+  std::memset(&m, 0, sizeof(m));
+  m.opts = bvht_LBVH_defaultOptions();
+  m.opts.mortonBits = mortonBits;
+  m.numFaces = n;
+  m.internal = &in;
+  m.lRes = true;
+  const int keyBytes = mortonBits == 63 ? 8 : 4;
+  in.useSplit = mortonBits == 63;
+  if (in.useSplit) {
+    in.split = makeSplitPlan(n);
+    m.sortScratchBytes = splitScratchBytes(in.split);
+  } else {
+    in.plan = makeSortPlan(n, keyBytes, (int)mortonBits);
+    m.sortScratchBytes = sortScratchBytes(in.plan);
+  }
+  in.boxBlocks = boxBlocks(n);
+  if (carve(nullptr, n, keyBytes, m, in) > kNoAllocBytes) {  // cannot happen for n <= kNoAllocMaxFaces
+    setError("bvht_LBVH_buildNoAlloc: %u faces need more than %zu bytes of scratch", n, kNoAllocBytes);
+    return false;
+  }
+  uint8_t *base = scratchAddress();
+  if (!base) return false;
+  carve(base, n, keyBytes, m, in);
+  // the scene-box reduction's completion counter starts at zero
+  if (!fillU32(reinterpret_cast<uint32_t *>(m.params), 0u, sizeof(BuildParams) / 4)) return false;
+  return buildAll(&m, mesh, bvh);
+}

@@ -249,6 +249,16 @@ bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvh
  * long groups of equal top bits overflowed.
  * 0xFFFFFFFF on a bad argument or a failed copy. */
 uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem);
+/* LBVH build that allocates nothing (no counterpart in the reference).  For
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc: the same
+ * build as bvht_LBVH_build with default options (mortonBits 30 or 63), its
+ * working memory taken from a library-owned device array of 28 MiB per GPU
+ * (held until the process ends).  Meshes of at most 2^20 faces; larger
+ * ones are refused with an error.  The caller passes the device mesh and a
+ * BVH allocated for it (LBVH_allocateBVH).  Calls that overlap on one GPU
+ * race (they share its array).  No hipMalloc or hipFree is called. */
+bool bvht_LBVH_buildNoAlloc(const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh, uint32_t mortonBits);
 /* Sorts the n (key, index) pairs of device keys[0..n) (keyBits 30: uint32
  * keys, 63: uint64 keys, each below 2^keyBits) by key then index, i.e. the
  * permutation of the reference's stable radix sort with index values
