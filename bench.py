@@ -72,11 +72,14 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
 def profile_build(args):
     """Counter runs (rocprofv3 --pmc): the timed LBVH builds only, then three
     calibration kernels of known bytes whose counters per byte scale the
-    build's FETCH_SIZE / WRITE_SIZE to bytes: a device-to-device copy of
-    256 MiB (reads and writes 256 MiB), a streaming read (a sum over
+    build's FETCH_SIZE / WRITE_SIZE to bytes (libbvht_calibrate.so): a copy
+    of 256 MiB (reads and writes 256 MiB), a streaming read (a sum over
     256 MiB) and a gather (every 128-byte row of 256 MiB read once, in a
-    random order, and written contiguously).  The last two run through
-    torch, used here only as a source of kernels."""
+    random order given by 4-byte row indices, and written contiguously)."""
+    import ctypes
+
+    import numpy as np
+
     import bvhtest_amd as bt
 
     L = bt.lib()
@@ -87,26 +90,29 @@ def profile_build(args):
     for _ in range(args.steps):
         builder.build(tree)
     L.LBVH_doCUDASyc()
+    cal = ctypes.CDLL(os.path.join(ROOT, "bvhtest_amd", "lib", "libbvht_calibrate.so"))
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    cal.calib_copy.argtypes, cal.calib_copy.restype = [vp, vp, sz], ctypes.c_bool
+    cal.calib_read.argtypes, cal.calib_read.restype = [vp, sz, vp], ctypes.c_bool
+    cal.calib_gather.argtypes, cal.calib_gather.restype = [vp, vp, sz, vp], ctypes.c_bool
     nbytes = 256 << 20
+    num_rows = nbytes // 128
     a, b = bt.DeviceBuffer(nbytes), bt.DeviceBuffer(nbytes)
+    partial = bt.DeviceBuffer(2048 * 4)
+    rows = np.random.default_rng(7).permutation(num_rows).astype(np.uint32)
+    rows_dev = bt.DeviceBuffer(rows.nbytes).upload(rows)  # every index < num_rows: reads stay in `a`
     for _ in range(3):
-        if not L.runMemcpy(b.ptr, a.ptr, nbytes, 3):
-            raise SystemExit("device copy failed")
-    L.LBVH_doCUDASyc()
-    import torch
-
-    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    rows = x.view(-1, 32)
-    perm = torch.randperm(rows.shape[0], device="cuda")
-    torch.cuda.synchronize()
+        if not cal.calib_copy(b.ptr, a.ptr, nbytes):
+            raise SystemExit("calibration copy failed")
     for _ in range(3):
-        x.sum()
+        if not cal.calib_read(a.ptr, nbytes, partial.ptr):
+            raise SystemExit("calibration read failed")
     for _ in range(3):
-        rows.index_select(0, perm)
-    torch.cuda.synchronize()
+        if not cal.calib_gather(a.ptr, rows_dev.ptr, num_rows, b.ptr):
+            raise SystemExit("calibration gather failed")
     print(json.dumps({"profile_build": True, "steps": args.steps, "copy_bytes": nbytes, "read_bytes": nbytes,
-                      "gather_rows": int(rows.shape[0]), "gather_row_bytes": 128,
-                      "gather_index_bytes": int(perm.numel() * perm.element_size())}), flush=True)
+                      "gather_rows": num_rows, "gather_row_bytes": 128,
+                      "gather_index_bytes": int(rows.nbytes)}), flush=True)
 
 
 def profile_batch(args):

@@ -206,6 +206,194 @@ __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict_
     if (out[k] < n) valsOut[out[k]] = s_v[tid + k * kSortBlock];
 }
 
+// Wide tiles: workgroups of 1024 threads x 17 keys (17,408 keys per tile),
+// used when the tiles of a pass fit the CUs in one round (244 tiles at 4.24M
+// keys: a dispatch ramp of 244 workgroups instead of 753).  Ranking and
+// scatter are those of k_onesweep (16 waves, 256 digit owners).  A 64-bit
+// key is ranked on the 32-bit word that holds the pass's digit (a digit
+// never straddles the words) and completed from its other word when staged,
+// which keeps the tile within 128 VGPRs.  The tile's global offsets are the
+// sums of every earlier tile's published per-digit counts, loaded by all
+// 1024 threads at once (four per digit, 16 loads in flight each) instead of
+// a chained look-back.  Tile ids come from `counter`, so every tile a
+// workgroup waits for has started; the wait for a count is bounded and
+// raises the error flag.
+template <typename K>
+__global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restrict__ keysIn,
+                                                             const uint32_t *__restrict__ valsIn,
+                                                             K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
+                                                             uint32_t n, uint32_t shift,
+                                                             const uint32_t *__restrict__ hist,
+                                                             uint32_t *__restrict__ status,
+                                                             uint32_t *__restrict__ counter) {
+  constexpr int WAVES = kWideBlock / 64;
+  constexpr int ITEMS = kWideItems;
+  constexpr uint32_t TILE = kWideTile;
+  constexpr uint32_t GROUPS = kWideBlock / kSortRadix;  // threads per digit in the prefix sums
+  constexpr uint32_t BATCH = 16;                        // predecessor counts per thread per round trip
+  __shared__ uint32_t s_wave[WAVES][kSortRadix];        // per-wave digit counters, then prefix partials
+  __shared__ uint32_t s_bstart[kSortRadix];
+  __shared__ uint32_t s_gstart[kSortRadix];
+  __shared__ uint32_t s_scan[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ K s_buf[TILE];  // the staged keys, then the values in the same words
+
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  for (uint32_t i = tid; i < (uint32_t)WAVES * kSortRadix; i += kWideBlock) (&s_wave[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * TILE + wave * 64 * ITEMS;
+  const bool full = (tile + 1) * TILE <= n;
+
+  constexpr bool WIDE = sizeof(K) == 8;
+  const uint32_t *words = reinterpret_cast<const uint32_t *>(keysIn);
+  const uint32_t dw = (WIDE && shift >= 32) ? 1u : 0u;
+  const uint32_t dshift = WIDE ? (shift & 31u) : shift;
+  uint32_t kw[ITEMS], rank[ITEMS];
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t idx = base + i * 64 + lane;
+      kw[i] = WIDE ? words[2 * (size_t)idx + dw] : (uint32_t)keysIn[idx];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t idx = base + i * 64 + lane;
+      kw[i] = idx < n ? (WIDE ? words[2 * (size_t)idx + dw] : (uint32_t)keysIn[idx]) : 0u;
+    }
+  }
+
+  // stable in-wave ranking, as in k_onesweep
+  uint32_t *cnt = &s_wave[wave][0];
+  const uint64_t lt = __lanemask_lt();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool valid = base + i * 64 + lane < n;
+    const uint32_t d = (kw[i] >> dshift) & (kSortRadix - 1);
+    const uint64_t peers = matchDigit<kSortRadixBits>(d, __ballot(valid));
+    const uint32_t lower = (uint32_t)__popcll(peers & lt);
+    const uint32_t c = __hip_atomic_load(cnt + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    rank[i] = c + lower;
+    if (valid && lower == 0)
+      __hip_atomic_store(cnt + d, c + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+
+  // digit owners (tid < 256): per-wave exclusive offsets, the tile's count,
+  // published at once
+  uint32_t total = 0;
+  if (tid < kSortRadix) {
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_wave[w][tid];
+      s_wave[w][tid] = total;
+      total += c;
+    }
+    storeRelaxedAgent(status + (size_t)tile * kSortRadix + tid, kFlagAgg | total);
+  }
+  const uint32_t hbase = blockExclusiveScan<kWideBlock>(tid < kSortRadix ? hist[tid] : 0u, s_scan);
+  const uint32_t bstart = blockExclusiveScan<kWideBlock>(total, s_scan);
+  if (tid < kSortRadix) s_bstart[tid] = bstart;
+  __syncthreads();
+
+  // keys to their tile positions; a 64-bit key's other word is loaded here
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t idx = base + i * 64 + lane;
+    if (idx < n) {
+      const uint32_t d = (kw[i] >> dshift) & (kSortRadix - 1);
+      rank[i] += s_bstart[d] + s_wave[wave][d];
+      K key;
+      if (WIDE) {
+        const uint32_t other = words[2 * (size_t)idx + (dw ^ 1u)];
+        key = (K)(dw ? (((uint64_t)kw[i] << 32) | other) : (((uint64_t)other << 32) | kw[i]));
+      } else {
+        key = (K)kw[i];
+      }
+      s_buf[rank[i]] = key;
+    }
+  }
+  uint32_t val[ITEMS];
+  if (!valsIn) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) val[i] = base + i * 64 + lane;
+  } else if (full) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) val[i] = valsIn[base + i * 64 + lane];
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t idx = base + i * 64 + lane;
+      val[i] = idx < n ? valsIn[idx] : 0u;
+    }
+  }
+  __syncthreads();  // s_wave is free from here: it holds the prefix partials
+
+  // offsets: the counts of every earlier tile, digit d summed by the GROUPS
+  // threads d, d + 256, ... over predecessors g, g + GROUPS, ...
+  {
+    const uint32_t d = tid & (kSortRadix - 1), g = tid >> kSortRadixBits;
+    uint32_t sum = 0, spins = 0;
+    for (uint32_t j0 = g; j0 < tile; j0 += GROUPS * BATCH) {
+      uint32_t sv[BATCH];
+#pragma unroll
+      for (uint32_t b = 0; b < BATCH; ++b) {
+        const uint32_t j = j0 + b * GROUPS;
+        sv[b] = j < tile ? loadRelaxedAgent(status + (size_t)j * kSortRadix + d) : kFlagAgg;
+      }
+#pragma unroll
+      for (uint32_t b = 0; b < BATCH; ++b) {
+        const uint32_t j = j0 + b * GROUPS;
+        uint32_t v = sv[b];
+        while ((v & ~kCountMask) == 0) {  // not yet published (j < tile: tile 0 never waits)
+          if (++spins > (1u << 24)) {
+            atomicOr(counter + 8, 1u);
+            v = kFlagAgg;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          v = loadRelaxedAgent(status + (size_t)j * kSortRadix + d);
+        }
+        sum += v & kCountMask;
+      }
+    }
+    s_wave[g][d] = sum;
+  }
+  __syncthreads();
+  if (tid < kSortRadix) {
+    uint32_t excl = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < GROUPS; ++g) excl += s_wave[g][tid];
+    s_gstart[tid] = hbase + excl;
+  }
+  __syncthreads();
+
+  const uint32_t tileCount = min(TILE, n - tile * TILE);
+  uint32_t out[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = tid + k * kWideBlock;
+    out[k] = n;
+    if (j < tileCount) {
+      const K kk = s_buf[j];
+      const uint32_t d = (uint32_t)(kk >> shift) & (kSortRadix - 1);
+      out[k] = s_gstart[d] + (j - s_bstart[d]);
+      if (out[k] >= n) out[k] = n;  // only reachable after a bounded-wait error
+      if (keysOut && out[k] < n) keysOut[out[k]] = kk;
+    }
+  }
+  __syncthreads();
+  uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (base + i * 64 + lane < n) s_v[rank[i]] = val[i];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k)
+    if (out[k] < n) valsOut[out[k]] = s_v[tid + k * kWideBlock];
+}
+
 }  // namespace
 
 SortPlan makeSortPlan(uint32_t n, int keyBytes, int keyBits) {
@@ -214,7 +402,10 @@ SortPlan makeSortPlan(uint32_t n, int keyBytes, int keyBits) {
   p.keyBytes = keyBytes;
   p.keyBits = keyBits;
   p.passes = (keyBits + kSortRadixBits - 1) / kSortRadixBits;
-  p.tile = kSortBlock * (keyBytes == 8 ? kSortItems64 : kSortItems32);
+  // wide tiles when they fit the CUs in one round of workgroups
+  const uint32_t wideTiles = n == 0 ? 0 : divUp(n, kWideTile);
+  p.wide = n > 0 && wideTiles <= (uint32_t)numCUs();
+  p.tile = p.wide ? kWideTile : kSortBlock * (keyBytes == 8 ? kSortItems64 : kSortItems32);
   p.numTiles = n == 0 ? 0 : divUp(n, p.tile);
   p.histWords = (size_t)p.passes * kSortRadix;
   p.counterWords = 16;
@@ -251,7 +442,13 @@ int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityVa
     bool last = pass == p.passes - 1;
     void *kout = (last && !keepKeys) ? nullptr : keys[cur ^ 1];
     uint32_t shift = pass * kSortRadixBits;
-    if (p.keyBytes == 8)
+    if (p.wide && p.keyBytes == 8)
+      k_onesweepWide<uint64_t><<<p.numTiles, kWideBlock, 0, stream()>>>(
+          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+    else if (p.wide)
+      k_onesweepWide<uint32_t><<<p.numTiles, kWideBlock, 0, stream()>>>(
+          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+    else if (p.keyBytes == 8)
       k_onesweep<uint64_t, kSortItems64><<<p.numTiles, kSortBlock, 0, stream()>>>(
           (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else

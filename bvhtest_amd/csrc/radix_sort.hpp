@@ -5,8 +5,10 @@
 // Stable, ascending, 8-bit digits.  One histogram of every pass is gathered
 // up front (fused into the producer of the keys where possible), then one
 // kernel per pass ranks a tile with wave64 ballots, publishes the tile's
-// per-digit counts, resolves its global offsets by decoupled look-back and
-// scatters through LDS so global writes are contiguous runs.
+// per-digit counts, resolves its global offsets (a full prefix over the
+// published counts when the tiles fit the CUs in one round, decoupled
+// look-back otherwise) and scatters through LDS so global writes are
+// contiguous runs.
 #pragma once
 
 #include "runtime.hpp"
@@ -18,6 +20,10 @@ constexpr int kSortRadix = 1 << kSortRadixBits;
 constexpr int kSortBlock = 256;
 constexpr int kSortItems32 = 20;  // keys per thread, 32-bit keys
 constexpr int kSortItems64 = 22;  // keys per thread, 64-bit keys: 753 tiles at 4.24M keys, one wave of workgroups
+// Wide tiles (k_onesweepWide): 1024 threads x 17 keys, 244 tiles at 4.24M keys.
+constexpr int kWideBlock = 1024;
+constexpr int kWideItems = 17;
+constexpr uint32_t kWideTile = kWideBlock * kWideItems;
 
 struct SortPlan {
   uint32_t n = 0;
@@ -26,6 +32,7 @@ struct SortPlan {
   int passes = 0;
   uint32_t tile = 0;
   uint32_t numTiles = 0;
+  bool wide = false;  // k_onesweepWide (tiles of kWideTile keys)
   // u32 word offsets into the scratch buffer
   size_t histWords = 0, counterWords = 0, statusWords = 0;
   size_t histOff = 0, counterOff = 0, statusOff = 0, totalWords = 0;

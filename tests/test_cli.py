@@ -6,6 +6,7 @@ are skipped with a warning, a JSON error ends the run with status 2, the
 first failing command ends the current input).
 """
 import json
+import re
 import os
 import subprocess
 import zlib
@@ -168,11 +169,31 @@ def test_full_gpu_pipeline(tmp_path, orc, bvh):
     p = write_cfg(tmp_path, base + opt + tail)
     r = cli("run", str(p))
     assert r.returncode == 0 and "[error]" not in r.stderr, r.stderr
+    # writeData: the key set of lib/io/WriteData.cpp:40-59; per-step SAH,
+    # skip count and final tree from the oracle's replay of the same steps
+    # (the device optimizer equals it after every chunk)
     data = json.loads((d / "rData_t.json").read_text())
-    assert len(data["optStepps"]) == 6 and data["numFaces"] == len(f)
+    assert set(data) == {"totalTime", "optStepps", "numSkipped", "numTotal", "numFaces", "numVert", "numBVHNodes"}
+    assert data["numFaces"] == len(f) and data["numVert"] == len(v) and data["numBVHNodes"] == 2 * len(f) - 1
+    assert [set(s) for s in data["optStepps"]] == [{"step", "sah", "duration"}] * 6
+    assert [s["step"] for s in data["optStepps"]] == list(range(6))
+    assert data["totalTime"] >= 0 and all(s["duration"] >= 0 for s in data["optStepps"])
+    _, _, lb, _ = orc.lbvh(v, f)
+    rc, rep_nodes, rep_sah, rep_stats = orc.bittner_gpuref(lb, 5.0, 4, 5, search=1)
+    assert rc == 0
     sah = [s["sah"] for s in data["optStepps"]]
+    np.testing.assert_allclose(sah, rep_sah, rtol=1e-5)
     assert sah[-1] < sah[0]
+    assert data["numSkipped"] == int(rep_stats[0]) + int(rep_stats[2])  # lock conflicts + contained
+    chunk = int(np.float32(5.0 / 100) * np.float32(len(lb))) // 4
+    assert data["numTotal"] == chunk * 4 * 5
     out = np.frombuffer((d / "out_bvh.bin").read_bytes(), dtype=orc.NODE_DTYPE)
     ml = orc.fix_levels(out.copy(), 0)
     total, err = orc.validate(out, 0, ml, v, f)
     assert total == 0, err
+    ref = rep_nodes.copy()
+    orc.fix_levels(ref, 0)
+    assert out.tobytes() == ref.tobytes()
+    # status prints the final tree's SAH (lib/base/StatusDump.cpp:31-56)
+    m = re.search(r"- SAH:\s+([0-9.eE+-]+)", r.stdout)
+    assert m and abs(float(m.group(1)) - orc.calc_sah(ref, 0)) <= 1e-5 * orc.calc_sah(ref, 0)

@@ -29,6 +29,10 @@ def cases(bits):
         "two_equal": np.array([7, 7]),
         "random_3000": rng.integers(0, hi, 3000, dtype=np.uint64),
         "random_1M": rng.integers(0, hi, 1 << 20, dtype=np.uint64),
+        # 256 tiles of 17,408 keys (the most the one-round tiles take on a
+        # 256-CU device), then one key more (tiles of 5,632 / 5,120 keys)
+        "random_256_wide_tiles": rng.integers(0, hi, 256 * 17408, dtype=np.uint64),
+        "random_past_wide_tiles": rng.integers(0, hi, 256 * 17408 + 1, dtype=np.uint64),
         "all_equal_20k": np.full(20000, hi // 3, np.uint64),
         "few_distinct": (rng.integers(0, 3, 50000).astype(np.uint64) << np.uint64(lo_bits + 2)),
         "one_bucket_spread": top(bits, 77) | rng.integers(0, 1 << lo_bits, 60000, dtype=np.uint64),
@@ -59,7 +63,8 @@ def cases(bits):
     return c
 
 
-CASES = ["one", "two_equal", "random_3000", "random_1M", "all_equal_20k", "few_distinct", "one_bucket_spread",
+CASES = ["one", "two_equal", "random_3000", "random_1M", "random_256_wide_tiles", "random_past_wide_tiles",
+         "all_equal_20k", "few_distinct", "one_bucket_spread",
          "one_bucket_narrow", "window_sized_buckets", "lds_limit_buckets", "sorted", "reversed", "clusters",
          "mixed_groups"]
 
