@@ -209,16 +209,21 @@ __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict_
 // Wide tiles: workgroups of 1024 threads x 17 keys (17,408 keys per tile),
 // used when the tiles of a pass fit the CUs in one round (244 tiles at 4.24M
 // keys: a dispatch ramp of 244 workgroups instead of 753).  Ranking and
-// scatter are those of k_onesweep (16 waves, 256 digit owners).  A 64-bit
-// key is ranked on the 32-bit word that holds the pass's digit (a digit
-// never straddles the words) and completed from its other word when staged,
-// which keeps the tile within 128 VGPRs.  The tile's global offsets are the
-// sums of every earlier tile's published per-digit counts, loaded by all
-// 1024 threads at once (four per digit, 16 loads in flight each) instead of
-// a chained look-back.  Tile ids come from `counter`, so every tile a
-// workgroup waits for has started; the wait for a count is bounded and
-// raises the error flag.
-template <typename K>
+// scatter are those of k_onesweep (16 waves, 256 digit owners).
+//  - Global accesses are buffer loads and stores with 32-bit byte offsets
+//    (the wide path only runs below 2^23 keys): one address register per
+//    access instead of a 64-bit pointer per item, which keeps the kernel
+//    within 128 VGPRs without spilling.  Loads past the end of an array read
+//    0 and stores past it are dropped by the buffer's range check.
+//  - A 64-bit key is loaded whole and ranked on the 32-bit word that holds
+//    the pass's digit (a digit never straddles the words).
+//  - The identity values of a first pass are made when they are staged.
+//  - The tile's global offsets are the sums of every earlier tile's
+//    published per-digit counts, loaded by all 1024 threads at once (four
+//    per digit, 16 loads in flight each) instead of a chained look-back.
+//    Tile ids come from `counter`, so every tile a workgroup waits for has
+//    started; the wait for a count is bounded and raises the error flag.
+template <typename K, bool IDENTITY>
 __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restrict__ keysIn,
                                                              const uint32_t *__restrict__ valsIn,
                                                              K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
@@ -231,7 +236,8 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   constexpr uint32_t TILE = kWideTile;
   constexpr uint32_t GROUPS = kWideBlock / kSortRadix;  // threads per digit in the prefix sums
   constexpr uint32_t BATCH = 16;                        // predecessor counts per thread per round trip
-  __shared__ uint32_t s_wave[WAVES][kSortRadix];        // per-wave digit counters, then prefix partials
+  constexpr bool WIDE = sizeof(K) == 8;
+  __shared__ uint32_t s_wave[WAVES][kSortRadix];  // per-wave digit counters, then prefix partials
   __shared__ uint32_t s_bstart[kSortRadix];
   __shared__ uint32_t s_gstart[kSortRadix];
   __shared__ uint32_t s_scan[WAVES];
@@ -243,25 +249,20 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   for (uint32_t i = tid; i < (uint32_t)WAVES * kSortRadix; i += kWideBlock) (&s_wave[0][0])[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint32_t base = tile * TILE + wave * 64 * ITEMS;
-  const bool full = (tile + 1) * TILE <= n;
+  const uint32_t first = tile * TILE + wave * 64 * ITEMS + lane;  // item i is key first + 64 i
 
-  constexpr bool WIDE = sizeof(K) == 8;
-  const uint32_t *words = reinterpret_cast<const uint32_t *>(keysIn);
-  const uint32_t dw = (WIDE && shift >= 32) ? 1u : 0u;
+  const BufferRsrc keys = bufferRsrc(keysIn, n * (uint32_t)sizeof(K));
+  const uint32_t dw = (WIDE && shift >= 32) ? 1u : 0u;  // the digit's word
   const uint32_t dshift = WIDE ? (shift & 31u) : shift;
-  uint32_t kw[ITEMS], rank[ITEMS];
-  if (full) {
+  uint32_t kw[ITEMS], rank[ITEMS], other[WIDE ? ITEMS : 1];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = base + i * 64 + lane;
-      kw[i] = WIDE ? words[2 * (size_t)idx + dw] : (uint32_t)keysIn[idx];
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = base + i * 64 + lane;
-      kw[i] = idx < n ? (WIDE ? words[2 * (size_t)idx + dw] : (uint32_t)keysIn[idx]) : 0u;
+  for (int i = 0; i < ITEMS; ++i) {
+    if (WIDE) {
+      const BufferU32x2 k2 = __builtin_amdgcn_raw_buffer_load_b64(keys, (first + i * 64) * 8u, 0, 0);
+      kw[i] = dw ? k2[1] : k2[0];
+      other[WIDE ? i : 0] = dw ? k2[0] : k2[1];
+    } else {
+      kw[i] = bufferLoadU32(keys, (first + i * 64) * 4u);
     }
   }
 
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   const uint64_t lt = __lanemask_lt();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const bool valid = base + i * 64 + lane < n;
+    const bool valid = first + i * 64 < n;
     const uint32_t d = (kw[i] >> dshift) & (kSortRadix - 1);
     const uint64_t peers = matchDigit<kSortRadixBits>(d, __ballot(valid));
     const uint32_t lower = (uint32_t)__popcll(peers & lt);
@@ -297,36 +298,36 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   if (tid < kSortRadix) s_bstart[tid] = bstart;
   __syncthreads();
 
-  // keys to their tile positions; a 64-bit key's other word is loaded here
+  // keys to their tile positions; rank[i] becomes the item's tile position
+  // (digits and item indices are recomputed here, not kept from the ranking)
+  uint32_t first2 = first;
+  recomputeAfter(first2);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t idx = base + i * 64 + lane;
-    if (idx < n) {
-      const uint32_t d = (kw[i] >> dshift) & (kSortRadix - 1);
-      rank[i] += s_bstart[d] + s_wave[wave][d];
-      K key;
-      if (WIDE) {
-        const uint32_t other = words[2 * (size_t)idx + (dw ^ 1u)];
-        key = (K)(dw ? (((uint64_t)kw[i] << 32) | other) : (((uint64_t)other << 32) | kw[i]));
-      } else {
-        key = (K)kw[i];
+  for (int i = 0; i < ITEMS; ++i) recomputeAfter(kw[i]);
+  if (WIDE) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (first2 + i * 64 < n) {
+        const uint32_t d = (kw[i] >> dshift) & (kSortRadix - 1);
+        rank[i] += s_bstart[d] + s_wave[wave][d];
+        s_buf[rank[i]] = (K)(dw ? (((uint64_t)kw[i] << 32) | other[WIDE ? i : 0]) : (((uint64_t)other[WIDE ? i : 0] << 32) | kw[i]));
       }
-      s_buf[rank[i]] = key;
     }
-  }
-  uint32_t val[ITEMS];
-  if (!valsIn) {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) val[i] = base + i * 64 + lane;
-  } else if (full) {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) val[i] = valsIn[base + i * 64 + lane];
   } else {
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t idx = base + i * 64 + lane;
-      val[i] = idx < n ? valsIn[idx] : 0u;
+      if (first2 + i * 64 < n) {
+        const uint32_t d = (kw[i] >> dshift) & (kSortRadix - 1);
+        rank[i] += s_bstart[d] + s_wave[wave][d];
+        s_buf[rank[i]] = (K)kw[i];
+      }
     }
+  }
+  uint32_t val[IDENTITY ? 1 : ITEMS];
+  if (!IDENTITY) {
+    const BufferRsrc vals = bufferRsrc(valsIn, n * 4u);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) val[IDENTITY ? 0 : i] = bufferLoadU32(vals, (first2 + i * 64) * 4u);
   }
   __syncthreads();  // s_wave is free from here: it holds the prefix partials
 
@@ -370,6 +371,8 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   __syncthreads();
 
   const uint32_t tileCount = min(TILE, n - tile * TILE);
+  const BufferRsrc kout = bufferRsrc(keysOut, keysOut ? n * (uint32_t)sizeof(K) : 0u);
+  const BufferRsrc vout = bufferRsrc(valsOut, n * 4u);
   uint32_t out[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
@@ -380,18 +383,20 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
       const uint32_t d = (uint32_t)(kk >> shift) & (kSortRadix - 1);
       out[k] = s_gstart[d] + (j - s_bstart[d]);
       if (out[k] >= n) out[k] = n;  // only reachable after a bounded-wait error
-      if (keysOut && out[k] < n) keysOut[out[k]] = kk;
+      if (keysOut && out[k] < n) bufferStore(kk, kout, out[k] * (uint32_t)sizeof(K));
     }
   }
   __syncthreads();
   uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
+  uint32_t first3 = first;
+  recomputeAfter(first3);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
-    if (base + i * 64 + lane < n) s_v[rank[i]] = val[i];
+    if (first3 + i * 64 < n) s_v[rank[i]] = IDENTITY ? first3 + i * 64 : val[IDENTITY ? 0 : i];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k)
-    if (out[k] < n) valsOut[out[k]] = s_v[tid + k * kWideBlock];
+    if (out[k] < n) bufferStore(s_v[tid + k * kWideBlock], vout, out[k] * 4u);
 }
 
 }  // namespace
@@ -402,9 +407,10 @@ SortPlan makeSortPlan(uint32_t n, int keyBytes, int keyBits) {
   p.keyBytes = keyBytes;
   p.keyBits = keyBits;
   p.passes = (keyBits + kSortRadixBits - 1) / kSortRadixBits;
-  // wide tiles when they fit the CUs in one round of workgroups
+  // wide tiles when they fit the CUs in one round of workgroups (their
+  // 32-bit byte offsets need n * 8 < 2^31: true for any device below 30,000 CUs)
   const uint32_t wideTiles = n == 0 ? 0 : divUp(n, kWideTile);
-  p.wide = n > 0 && wideTiles <= (uint32_t)numCUs();
+  p.wide = n > 0 && wideTiles <= (uint32_t)numCUs() && (uint64_t)n * 8 < (1ull << 31);
   p.tile = p.wide ? kWideTile : kSortBlock * (keyBytes == 8 ? kSortItems64 : kSortItems32);
   p.numTiles = n == 0 ? 0 : divUp(n, p.tile);
   p.histWords = (size_t)p.passes * kSortRadix;
@@ -442,11 +448,17 @@ int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityVa
     bool last = pass == p.passes - 1;
     void *kout = (last && !keepKeys) ? nullptr : keys[cur ^ 1];
     uint32_t shift = pass * kSortRadixBits;
-    if (p.wide && p.keyBytes == 8)
-      k_onesweepWide<uint64_t><<<p.numTiles, kWideBlock, 0, stream()>>>(
+    if (p.wide && p.keyBytes == 8 && !vin)
+      k_onesweepWide<uint64_t, true><<<p.numTiles, kWideBlock, 0, stream()>>>(
           (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+    else if (p.wide && p.keyBytes == 8)
+      k_onesweepWide<uint64_t, false><<<p.numTiles, kWideBlock, 0, stream()>>>(
+          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+    else if (p.wide && !vin)
+      k_onesweepWide<uint32_t, true><<<p.numTiles, kWideBlock, 0, stream()>>>(
+          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else if (p.wide)
-      k_onesweepWide<uint32_t><<<p.numTiles, kWideBlock, 0, stream()>>>(
+      k_onesweepWide<uint32_t, false><<<p.numTiles, kWideBlock, 0, stream()>>>(
           (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else if (p.keyBytes == 8)
       k_onesweep<uint64_t, kSortItems64><<<p.numTiles, kSortBlock, 0, stream()>>>(

@@ -17,6 +17,31 @@ __device__ __forceinline__ uint32_t loadRelaxedAgent(const uint32_t *p) {
   return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Hides a register's value from the optimiser (no instruction), so values
+// derived from it are recomputed after this point instead of being kept
+// live (or spilled) from before it.
+__device__ __forceinline__ void recomputeAfter(uint32_t &x) { asm volatile("" : "+v"(x)); }
+
+// Raw buffer accesses (gfx9 buffer resource: base, `bytes` records, 32-bit
+// data format in word 3).  Offsets are 32-bit byte offsets in a VGPR (plus
+// the immediate the compiler folds in); both take part in the range check,
+// so a load at or past `bytes` returns 0 and such a store is dropped.
+// `bytes` must stay below 2^31.
+typedef __amdgpu_buffer_rsrc_t BufferRsrc;
+typedef unsigned int BufferU32x2 __attribute__((vector_size(8)));
+__device__ __forceinline__ BufferRsrc bufferRsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t bufferLoadU32(BufferRsrc r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ void bufferStore(uint32_t v, BufferRsrc r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
+}
+__device__ __forceinline__ void bufferStore(uint64_t v, BufferRsrc r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b64(BufferU32x2{(uint32_t)v, (uint32_t)(v >> 32)}, r, off, 0, 0);
+}
+
 // Agent-scope fences for publishing data through a relaxed counter RMW.
 // In a loop that stores and then increments a counter on the next trip, an
 // acq_rel RMW at the loop head is lowered here to `buffer_wbl2` + the atomic
@@ -37,12 +62,16 @@ __device__ __forceinline__ uint32_t divUpDev(uint32_t a, uint32_t b) { return (a
 template <int BITS>
 __device__ __forceinline__ uint64_t matchDigit(uint32_t d, uint64_t peers) {
   uint32_t lo = (uint32_t)peers, hi = (uint32_t)(peers >> 32);
+  uint32_t t[BITS];
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) t[b] = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
+  if constexpr (BITS == 8)  // ballots of t itself: one compare per bit instead of a shift and a compare
+    asm("" : "+v"(t[0]), "+v"(t[1]), "+v"(t[2]), "+v"(t[3]), "+v"(t[4]), "+v"(t[5]), "+v"(t[6]), "+v"(t[7]));
 #pragma unroll
   for (int b = 0; b < BITS; ++b) {
-    const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
-    const uint64_t m = __ballot(t != 0u);
-    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)m, t, 0x90);
-    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(m >> 32), t, 0x90);
+    const uint64_t m = __ballot(t[b] != 0u);
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)m, t[b], 0x90);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(m >> 32), t[b], 0x90);
   }
   return ((uint64_t)hi << 32) | lo;
 }
