@@ -736,36 +736,37 @@ __global__ __launch_bounds__(256) void k_refitApply(const uint32_t *__restrict__
   for (uint32_t steps = 0; steps < N; ++steps) {
     // The topology does not change during the refit, so its load goes out
     // beside the count (the chain of last arrivals near the root is serial:
-    // each level costs the count, the child boxes and the release).  Relaxed
+    // each level costs the count, the child boxes and the stores).  Relaxed
     // atomic loads keep the compiler from sinking them below the count.
     const uint32_t p = loadRelaxedAgent(&nodes[node].parent), l = loadRelaxedAgent(&nodes[node].left),
                    r = loadRelaxedAgent(&nodes[node].right);
-    // release of the boxes written on the previous trip: fence after the
-    // stores, then a relaxed count (see fenceReleaseAgent)
     const uint32_t left = __hip_atomic_fetch_sub(cnt + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (left != 1) return;
-    fenceAcquireAgent();
-    bvht_AABB b = nodes[l].bbox;
-    const bvht_AABB br = nodes[r].bbox;
+    // Boxes and features written by this walk move between XCDs through
+    // device-coherent accesses: the loads below read past the L2 (issued
+    // after the count returned), the stores write through it and are waited
+    // for before the next count.  This replaces an acquire fence (an L2
+    // invalidation) and a release fence (an L2 write-back) per level.
+    bvht_AABB b = loadBoxAgent(nodes + l);
+    const bvht_AABB br = loadBoxAgent(nodes + r);
     // the children's subtree features load with their boxes, ahead of the
     // node's stores (which may alias them), not one round trip later
     uint32_t cl = 0, cr = 0;
     float sl = 0.0f, sr = 0.0f;
     if (leafCnt) {
-      cl = leafCnt[l];
-      cr = leafCnt[r];
-      sl = subSA[l];
-      sr = subSA[r];
+      cl = loadRelaxedAgent(leafCnt + l);
+      cr = loadRelaxedAgent(leafCnt + r);
+      sl = loadRelaxedAgent(subSA + l);
+      sr = loadRelaxedAgent(subSA + r);
     }
     mergeInto(b, br);
-    nodes[node].bbox = b;
     const float sa = surfaceArea(b);
-    nodes[node].surfaceArea = sa;
+    storeBoxAgent(nodes + node, b, sa);
     if (leafCnt) {
-      leafCnt[node] = cl + cr;
-      subSA[node] = sl + sr + sa;
+      storeRelaxedAgent(leafCnt + node, cl + cr);
+      storeRelaxedAgent(subSA + node, sl + sr + sa);
     }
-    fenceReleaseAgent();
+    waitVectorMemory();
     if (p == node) return;
     node = p;
   }

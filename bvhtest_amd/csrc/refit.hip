@@ -1,10 +1,11 @@
 // Generic bottom-up refit (see refit.hpp).
 //
-// Arrival counting uses a relaxed agent-scope counter between explicit
-// release / acquire fences: the box a thread writes for a node must be
-// visible to the sibling's thread, which may run on another XCD with its own
-// L2.  Every walk is bounded by the node
-// count, so a malformed tree cannot hang the kernel.
+// Arrival counting uses a relaxed agent-scope counter; the box a thread
+// writes for a node must be visible to the sibling's thread, which may run
+// on another XCD with its own L2, so boxes move through device-coherent
+// loads and stores (loadBoxAgent / storeBoxAgent) and the stores are waited
+// for before the next count.  Every walk is bounded by the node count, so a
+// malformed tree cannot hang the kernel.
 #include "refit.hpp"
 #include "wave.hpp"
 
@@ -22,18 +23,16 @@ __global__ __launch_bounds__(256) void k_refitUp(bvht_BVHNode *__restrict__ node
   if (node == leaf || node >= numNodes) return;
   for (uint32_t steps = 0; steps < numNodes; ++steps) {
     // the leaf boxes come from an earlier kernel; every box written below is
-    // released before the next arrival count (fence after the stores)
+    // complete (waited for) before the next arrival count.  The topology is
+    // not written here: its loads go out beside the count.
+    const uint32_t l = loadRelaxedAgent(&nodes[node].left), r = loadRelaxedAgent(&nodes[node].right);
+    const uint32_t parent = loadRelaxedAgent(&nodes[node].parent);
     const uint32_t seen = __hip_atomic_fetch_add(counters + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (seen == 0) return;
-    fenceAcquireAgent();
-    const uint32_t l = nodes[node].left, r = nodes[node].right;
-    const bvht_AABB a = nodes[l].bbox, b = nodes[r].bbox;
-    bvht_AABB m = a;
-    mergeInto(m, b);
-    nodes[node].bbox = m;
-    nodes[node].surfaceArea = surfaceArea(m);
-    fenceReleaseAgent();
-    const uint32_t parent = nodes[node].parent;
+    bvht_AABB m = loadBoxAgent(nodes + l);
+    mergeInto(m, loadBoxAgent(nodes + r));
+    storeBoxAgent(nodes + node, m, surfaceArea(m));
+    waitVectorMemory();
     if (parent == node || parent >= numNodes) return;
     node = parent;
   }

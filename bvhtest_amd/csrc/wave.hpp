@@ -16,6 +16,18 @@ __device__ __forceinline__ void storeRelaxedAgent(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ uint32_t loadRelaxedAgent(const uint32_t *p) {
   return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ float loadRelaxedAgent(const float *p) {
+  return __uint_as_float(loadRelaxedAgent(reinterpret_cast<const uint32_t *>(p)));
+}
+__device__ __forceinline__ void storeRelaxedAgent(float *p, float v) {
+  storeRelaxedAgent(reinterpret_cast<uint32_t *>(p), __float_as_uint(v));
+}
+// Waits until this wave's outstanding vector memory operations are done
+// (s_waitcnt vmcnt(0)).  After device-coherent stores (the relaxed
+// agent-scope stores above write through the XCD's L2) this orders them
+// before a later atomic as a release does, without the L2 write-back of
+// every dirty line that a release fence performs.
+__device__ __forceinline__ void waitVectorMemory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Hides a register's value from the optimiser (no instruction), so values
 // derived from it are recomputed after this point instead of being kept
