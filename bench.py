@@ -39,8 +39,9 @@ SORT_STAGES = ("k_msdScan", "k_msdScatter", "k_msdWindows", "k_msdSplit", "k_msd
 def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
     """Algorithmic HBM bytes of one launch, per the byte model of the build
     (n triangles, nv vertices, each vertex counted once):
-    scene box: faces 12 n + vertices 12 nv; Morton: the same plus the key
-    write; MSD scatter: key read, key + index written; onesweep pass: key +
+    scene box: faces 12 n + vertices 12 nv (from the triangles) or vertices
+    12 nv (from every vertex, the full build's pass); Morton: faces 12 n +
+    vertices 12 nv plus the key write; MSD scatter: key read, key + index written; onesweep pass: key +
     value read and written (first pass reads no value, last pass writes no
     key; 63-bit keys: the 5 passes over bits 24-62, all writing keys); group
     rank: key 8 + index 4 read, index 4 written (each group member read once
@@ -52,6 +53,8 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
     kb = 8 if morton_bits == 63 else 4
     if name == "k_sceneBox":
         return 12 * n + 12 * nv
+    if name == "k_vertBox":
+        return 12 * nv
     if name in ("k_morton", "k_mortonMsd"):
         return 12 * n + 12 * nv + kb * n
     if name == "k_msdScatter":
@@ -228,7 +231,7 @@ def run_bench(args):
 
     # per-stage and end-to-end fractions of the HBM peak: modelled bytes over
     # the stage's measured time (kernels without a byte model add time only)
-    stage_groups = {"encode": ("k_sceneBox", "k_morton"), "sort": SORT_STAGES, "build": ("k_buildLevel",)}
+    stage_groups = {"encode": ("k_sceneBox", "k_vertBox", "k_morton"), "sort": SORT_STAGES, "build": ("k_buildLevel",)}
     stage_roofline = {}
     total_bytes = 0
     for g, prefixes in stage_groups.items():

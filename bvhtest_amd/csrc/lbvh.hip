@@ -40,6 +40,85 @@ constexpr uint32_t kBoxBlocksMax = 1024;
 
 __device__ __forceinline__ bvht_vec3 ldVec3(const bvht_vec3 *p) { return *p; }
 
+// Block reduction of per-thread boxes into partials[blockIdx.x]; returns,
+// to every thread of the block, whether this block finished last (every
+// other block's partial is then visible).  `done` is reset by the caller.
+__device__ bool boxPartials(const float mn[3], const float mx[3], float *__restrict__ partials, uint32_t *done) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ float s_red[6][kBoxBlock / 64];
+  __shared__ bool s_last;
+  for (int a = 0; a < 3; ++a) {
+    float x = waveMin(mn[a]), y = waveMax(mx[a]);
+    if (lane == 0) {
+      s_red[a][wave] = x;
+      s_red[3 + a][wave] = y;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      float x = s_red[a][0], y = s_red[3 + a][0];
+      for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+        x = mergeMin(x, s_red[a][w]);
+        y = mergeMax(y, s_red[3 + a][w]);
+      }
+      // device-coherent stores, complete before the count: no L2-wide
+      // write-back (a release fence here wrote back every dirty line, e.g.
+      // the keys of the Morton pass, once per block)
+      storeRelaxedAgent(partials + blockIdx.x * 6 + a, x);
+      storeRelaxedAgent(partials + blockIdx.x * 6 + 3 + a, y);
+    }
+    waitVectorMemory();
+    s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  return s_last;
+}
+
+// The last block's union of every block's partial, starting from the
+// all-zero box (LBVH_CUDA.cu:104-106); the result is in thread 0's out.
+__device__ void boxFinal(const float *__restrict__ partials, float out[6]) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ float s_red[6][kBoxBlock / 64];
+  float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+  for (uint32_t b = tid; b < gridDim.x; b += blockDim.x)  // device-coherent loads after the count
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = mergeMin(mn[a], loadRelaxedAgent(partials + b * 6 + a));
+      mx[a] = mergeMax(mx[a], loadRelaxedAgent(partials + b * 6 + 3 + a));
+    }
+  __syncthreads();
+  for (int a = 0; a < 3; ++a) {
+    float x = waveMin(mn[a]), y = waveMax(mx[a]);
+    if (lane == 0) {
+      s_red[a][wave] = x;
+      s_red[3 + a][wave] = y;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int a = 0; a < 3; ++a) {
+      float x = s_red[a][0], y = s_red[3 + a][0];
+      for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+        x = mergeMin(x, s_red[a][w]);
+        y = mergeMax(y, s_red[3 + a][w]);
+      }
+      out[a] = x;
+      out[3 + a] = y;
+    }
+  }
+}
+
+// LBVH_calcMortonCodes host part (LBVH_CUDA.cu:171-180), f32, IEEE division.
+__device__ void setSceneBox(BuildParams *prm, const float box[6]) {
+  for (int a = 0; a < 3; ++a) {
+    prm->sceneMin[a] = box[a];
+    prm->sceneMax[a] = box[3 + a];
+    const float off = 0.0f - box[a];
+    prm->offset[a] = off;
+    prm->scale[a] = __fdiv_rn(1.0f, box[3 + a] + off);
+  }
+}
+
 __global__ __launch_bounds__(kBoxBlock) void k_sceneBox(const bvht_vec3 *__restrict__ vert,
                                                         const bvht_Triangle *__restrict__ faces, uint32_t n,
                                                         BuildParams *__restrict__ prm, float *__restrict__ partials,
@@ -61,90 +140,164 @@ __global__ __launch_bounds__(kBoxBlock) void k_sceneBox(const bvht_vec3 *__restr
     mx[1] = mergeMax(mx[1], bx.minMax[1].y);
     mx[2] = mergeMax(mx[2], bx.minMax[1].z);
   }
+  if (!boxPartials(mn, mx, partials, &prm->done)) return;
+  float box[6];
+  boxFinal(partials, box);
+  if (tid == 0) {
+    setSceneBox(prm, box);
+    prm->done = 0;
+  }
+}
+
+// Scene box from every vertex instead of every triangle (the full build's
+// first pass).  A triangle box is its vertices' min - FLT_EPSILON /
+// max + FLT_EPSILON; f32 rounding is monotone, so the union over triangles
+// equals the min / max over the referenced vertices shifted by FLT_EPSILON
+// (a zero's sign is lost in the shift), and over all vertices when every
+// vertex is referenced.  Unreferenced vertices outside the triangles' box
+// make this box larger: k_morton<..., true> also reduces the triangles' box
+// and k_mortonFix compares the two and repairs the keys if they differ.
+//
+// Neither pass ends in a last-block reduction (one global atomic per block
+// on one counter, then a serial tail): each block writes its partial box,
+// and every block of the next kernel reduces all partials itself.
+__device__ __forceinline__ void shiftedBox(float mn[3], float mx[3]) {
+  for (int a = 0; a < 3; ++a) {  // -+FLT_EPSILON, then the origin (LBVH_CUDA.cu:104-106)
+    mn[a] = mergeMin(0.0f, mn[a] - kFltEpsilon);
+    mx[a] = mergeMax(0.0f, mx[a] + kFltEpsilon);
+  }
+}
+
+// Block reduction of per-thread boxes; thread 0 writes partials[blockIdx.x].
+__device__ void writeBoxPartial(const float mn[3], const float mx[3], float *__restrict__ partials) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ float s_red[6][kBoxBlock / 64];
-  __shared__ bool s_last;
   for (int a = 0; a < 3; ++a) {
-    float x = waveMin(mn[a]), y = waveMax(mx[a]);
+    const float x = waveMin(mn[a]), y = waveMax(mx[a]);
     if (lane == 0) {
       s_red[a][wave] = x;
       s_red[3 + a][wave] = y;
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    for (int a = 0; a < 3; ++a) {
-      float x = s_red[a][0], y = s_red[3 + a][0];
-      for (int w = 1; w < kBoxBlock / 64; ++w) {
-        x = mergeMin(x, s_red[a][w]);
-        y = mergeMax(y, s_red[3 + a][w]);
-      }
-      partials[blockIdx.x * 6 + a] = x;
-      partials[blockIdx.x * 6 + 3 + a] = y;
-    }
-    __threadfence();
-    s_last = atomicAdd(&prm->done, 1u) == gridDim.x - 1;
+  if (tid < 6) {
+    float x = s_red[tid][0];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) x = tid < 3 ? mergeMin(x, s_red[tid][w]) : mergeMax(x, s_red[tid][w]);
+    partials[blockIdx.x * 6 + tid] = x;
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int a = 0; a < 3; ++a) {
-    mn[a] = 0.0f;
-    mx[a] = 0.0f;
-  }
-  for (uint32_t b = tid; b < gridDim.x; b += kBoxBlock)
+}
+
+// Union of `count` partial boxes (written by an earlier kernel), starting
+// from the all-zero box; every thread of the block receives it.
+__device__ void readBoxPartials(const float *__restrict__ partials, uint32_t count, float box[6]) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ float s_red[6][kBoxBlock / 64];
+  __shared__ float s_box[6];
+  float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+  for (uint32_t b = tid; b < count; b += blockDim.x)
     for (int a = 0; a < 3; ++a) {
       mn[a] = mergeMin(mn[a], partials[b * 6 + a]);
       mx[a] = mergeMax(mx[a], partials[b * 6 + 3 + a]);
     }
-  __syncthreads();
   for (int a = 0; a < 3; ++a) {
-    float x = waveMin(mn[a]), y = waveMax(mx[a]);
+    const float x = waveMin(mn[a]), y = waveMax(mx[a]);
     if (lane == 0) {
       s_red[a][wave] = x;
       s_red[3 + a][wave] = y;
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    for (int a = 0; a < 3; ++a) {
-      float x = s_red[a][0], y = s_red[3 + a][0];
-      for (int w = 1; w < kBoxBlock / 64; ++w) {
-        x = mergeMin(x, s_red[a][w]);
-        y = mergeMax(y, s_red[3 + a][w]);
-      }
-      prm->sceneMin[a] = x;
-      prm->sceneMax[a] = y;
-      // LBVH_calcMortonCodes host part (LBVH_CUDA.cu:171-180), f32, IEEE division
-      float off = 0.0f - x;
-      prm->offset[a] = off;
-      prm->scale[a] = __fdiv_rn(1.0f, y + off);
-    }
-    prm->done = 0;
+  if (tid < 6) {
+    float x = s_red[tid][0];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) x = tid < 3 ? mergeMin(x, s_red[tid][w]) : mergeMax(x, s_red[tid][w]);
+    s_box[tid] = x;
   }
+  __syncthreads();
+  for (int a = 0; a < 6; ++a) box[a] = s_box[a];
+}
+
+// LBVH_calcMortonCodes host part (LBVH_CUDA.cu:171-180), f32, IEEE division.
+__device__ __forceinline__ void mortonMapping(const float box[6], bvht_vec3 &off, bvht_vec3 &sc) {
+  off = {0.0f - box[0], 0.0f - box[1], 0.0f - box[2]};
+  sc = {__fdiv_rn(1.0f, box[3] + off.x), __fdiv_rn(1.0f, box[4] + off.y), __fdiv_rn(1.0f, box[5] + off.z)};
+}
+
+__global__ __launch_bounds__(kBoxBlock) void k_vertBox(const bvht_vec3 *__restrict__ vert, uint32_t nv,
+                                                       float *__restrict__ partials, uint32_t *__restrict__ zeroPtr,
+                                                       size_t zeroWords) {
+  const uint32_t tid = threadIdx.x;
+  for (size_t i = (size_t)blockIdx.x * kBoxBlock + tid; i < zeroWords; i += (size_t)gridDim.x * kBoxBlock) zeroPtr[i] = 0;
+  float mn[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, mx[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+#pragma unroll 4
+  for (uint32_t i = blockIdx.x * kBoxBlock + tid; i < nv; i += gridDim.x * kBoxBlock) {
+    const bvht_vec3 p = vert[i];
+    mn[0] = fminf(mn[0], p.x);
+    mn[1] = fminf(mn[1], p.y);
+    mn[2] = fminf(mn[2], p.z);
+    mx[0] = fmaxf(mx[0], p.x);
+    mx[1] = fmaxf(mx[1], p.y);
+    mx[2] = fmaxf(mx[2], p.z);
+  }
+  shiftedBox(mn, mx);
+  writeBoxPartial(mn, mx, partials);
 }
 
 // Morton keys plus the histograms of the digit passes FIRST..PASSES-1 (the
 // passes the sort runs).  Workgroups of kMortonBlock threads: 16 waves per
 // workgroup hide the face -> vertex load chain while the grid stays at two
 // workgroups per CU (one global flush of the histograms per workgroup).
+// VERTEX: the mapping comes from k_vertBox's partials (reduced by every
+// block; block 0 records the box in prm) and the pass also writes its
+// blocks' partial triangle boxes (triParts) for k_mortonFix.  Otherwise the
+// mapping is prm's (the staged calls).
 constexpr int kMortonBlock = 1024;
-template <typename K, int PASSES, int FIRST>
+static_assert(kMortonBlock == kBoxBlock, "the encode passes share the box reductions");
+template <typename K, int PASSES, int FIRST, bool VERTEX>
 __global__ __launch_bounds__(kMortonBlock) void k_morton(const bvht_vec3 *__restrict__ vert,
                                                          const bvht_Triangle *__restrict__ faces, uint32_t n,
-                                                         const BuildParams *__restrict__ prm, K *__restrict__ keys,
-                                                         uint32_t *__restrict__ hist) {
+                                                         BuildParams *__restrict__ prm, K *__restrict__ keys,
+                                                         uint32_t *__restrict__ hist,
+                                                         const float *__restrict__ vertParts, uint32_t numVertParts,
+                                                         float *__restrict__ triParts) {
   __shared__ uint32_t s_hist[PASSES][kSortRadix];
   for (uint32_t j = threadIdx.x; j < PASSES * kSortRadix; j += kMortonBlock) (&s_hist[0][0])[j] = 0;
-  __syncthreads();
-  const bvht_vec3 off = {prm->offset[0], prm->offset[1], prm->offset[2]};
-  const bvht_vec3 sc = {prm->scale[0], prm->scale[1], prm->scale[2]};
+  bvht_vec3 off, sc;
+  if (VERTEX) {
+    float box[6];
+    readBoxPartials(vertParts, numVertParts, box);
+    mortonMapping(box, off, sc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      for (int a = 0; a < 3; ++a) {
+        prm->sceneMin[a] = box[a];
+        prm->sceneMax[a] = box[3 + a];
+        prm->offset[a] = (&off.x)[a];
+        prm->scale[a] = (&sc.x)[a];
+      }
+    }
+  } else {
+    __syncthreads();
+    off = {prm->offset[0], prm->offset[1], prm->offset[2]};
+    sc = {prm->scale[0], prm->scale[1], prm->scale[2]};
+  }
+  // VERTEX: min / max of the referenced vertices, shifted at the end like
+  // k_vertBox's (the triangles' box, by the same monotonicity)
+  float mn[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, mx[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
   // wave-uniform trips (the histogram adds use ballots)
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t base = (blockIdx.x * (kMortonBlock / 64) + wave) * 64; base < n; base += gridDim.x * kMortonBlock) {
     const uint32_t i = base + lane;
     const bool valid = i < n;
     const bvht_Triangle t = faces[valid ? i : base];
-    const bvht_vec3 c = centroid(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+    const bvht_vec3 va = ldVec3(vert + t.v1), vb = ldVec3(vert + t.v2), vc = ldVec3(vert + t.v3);
+    const bvht_vec3 c = centroid(va, vb, vc);
+    if (VERTEX) {  // a repeat of triangle `base` for a lane past the end changes no min/max
+      mn[0] = fminf(mn[0], fminf(fminf(va.x, vb.x), vc.x));
+      mn[1] = fminf(mn[1], fminf(fminf(va.y, vb.y), vc.y));
+      mn[2] = fminf(mn[2], fminf(fminf(va.z, vb.z), vc.z));
+      mx[0] = fmaxf(mx[0], fmaxf(fmaxf(va.x, vb.x), vc.x));
+      mx[1] = fmaxf(mx[1], fmaxf(fmaxf(va.y, vb.y), vc.y));
+      mx[2] = fmaxf(mx[2], fmaxf(fmaxf(va.z, vb.z), vc.z));
+    }
     K code;
     if constexpr (sizeof(K) == 8)
       code = morton63(c, off, sc);
@@ -154,6 +307,57 @@ __global__ __launch_bounds__(kMortonBlock) void k_morton(const bvht_vec3 *__rest
 #pragma unroll
     for (int p = FIRST; p < PASSES; ++p)
       waveHistAdd(s_hist[p], (uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1), valid);
+  }
+  __syncthreads();
+  for (uint32_t j = FIRST * kSortRadix + threadIdx.x; j < PASSES * kSortRadix; j += kMortonBlock) {
+    const uint32_t c = (&s_hist[0][0])[j];
+    if (c) atomicAdd(&hist[j], c);
+  }
+  if (!VERTEX) return;
+  shiftedBox(mn, mx);
+  writeBoxPartial(mn, mx, triParts);
+}
+
+// Compares the triangles' box (k_morton's partials) with the vertex box the
+// keys were made from (prm, from k_morton's block 0); when they differ
+// (unreferenced vertices outside the triangles' box) every key is
+// recomputed with the triangles' mapping and the histograms are corrected
+// by removing the old key's digits and adding the new one's.  Otherwise it
+// exits after the comparison.  prm is only read here (every block compares
+// against it).
+template <typename K, int PASSES, int FIRST>
+__global__ __launch_bounds__(kMortonBlock) void k_mortonFix(const bvht_vec3 *__restrict__ vert,
+                                                            const bvht_Triangle *__restrict__ faces, uint32_t n,
+                                                            BuildParams *__restrict__ prm, K *__restrict__ keys,
+                                                            uint32_t *__restrict__ hist,
+                                                            const float *__restrict__ triParts, uint32_t numTriParts) {
+  __shared__ uint32_t s_hist[PASSES][kSortRadix];  // net change per bin (mod 2^32)
+  float box[6];
+  readBoxPartials(triParts, numTriParts, box);
+  bool same = true;
+  for (int a = 0; a < 3; ++a)
+    same = same && __float_as_uint(box[a]) == __float_as_uint(prm->sceneMin[a]) &&
+           __float_as_uint(box[3 + a]) == __float_as_uint(prm->sceneMax[a]);
+  if (same) return;  // block-uniform
+  bvht_vec3 off, sc;
+  mortonMapping(box, off, sc);
+  for (uint32_t j = threadIdx.x; j < PASSES * kSortRadix; j += kMortonBlock) (&s_hist[0][0])[j] = 0;
+  __syncthreads();  // every block has compared before block 0 records the box below
+  for (uint32_t i = blockIdx.x * kMortonBlock + threadIdx.x; i < n; i += gridDim.x * kMortonBlock) {
+    const bvht_Triangle t = faces[i];
+    const bvht_vec3 c = centroid(ldVec3(vert + t.v1), ldVec3(vert + t.v2), ldVec3(vert + t.v3));
+    K code;
+    if constexpr (sizeof(K) == 8)
+      code = morton63(c, off, sc);
+    else
+      code = morton30(c, off, sc);
+    const K old = keys[i];
+    keys[i] = code;
+#pragma unroll
+    for (int p = FIRST; p < PASSES; ++p) {
+      atomicAdd(&s_hist[p][(uint32_t)(old >> (p * kSortRadixBits)) & (kSortRadix - 1)], 0xFFFFFFFFu);
+      atomicAdd(&s_hist[p][(uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1)], 1u);
+    }
   }
   __syncthreads();
   for (uint32_t j = FIRST * kSortRadix + threadIdx.x; j < PASSES * kSortRadix; j += kMortonBlock) {
@@ -332,6 +536,13 @@ uint32_t boxBlocks(uint32_t n) {
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
+// The vertex-box pass: two workgroups per CU.
+uint32_t vertBlocks(uint32_t nv) {
+  uint32_t b = divUp((uint64_t)nv + 1, kBoxBlock);
+  uint32_t cap = std::min<uint32_t>(kBoxBlocksMax, (uint32_t)numCUs() * 2);
+  return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
 // Every Morton workgroup ends with one global atomic per non-empty bin of
 // every pass into the same 8 x 256 counters, so the grid is kept to two
 // workgroups per CU (each then covers ~8K triangles at 4.24M triangles).
@@ -348,7 +559,9 @@ size_t sortZeroWords(const Internal *in) {
   return in->plan.totalWords;
 }
 
-bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
+// check: the keys are made from k_vertBox's box; verify it against the
+// triangles' box and repair the keys if it differs (k_mortonFix).
+bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bool check) {
   Internal *in = internalOf(mem);
   uint32_t n = mesh->numFaces;
   if (in->useMsd) {
@@ -366,13 +579,38 @@ bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh) {
   }
   uint32_t *hist = in->useSplit ? splitHist(in->split, (uint32_t *)mem->sortScratch)
                                 : sortHist(in->plan, (uint32_t *)mem->sortScratch);
-  if (mem->opts.mortonBits == 63)
-    k_morton<uint64_t, 8, kSplitFirstPass><<<mortonBlocks(n), kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, (const BuildParams *)mem->params, (uint64_t *)mem->keys[0], hist);
+  BuildParams *prm = (BuildParams *)mem->params;
+  const uint32_t blocks = mortonBlocks(n);
+  // vertex-box path: k_vertBox's partials in the first half of in->partials,
+  // the triangles' partials in the second
+  const float *vparts = in->partials;
+  const uint32_t nvparts = vertBlocks(mesh->numVert);
+  float *tparts = in->partials + (size_t)kBoxBlocksMax * 6;
+  if (mem->opts.mortonBits == 63 && check)
+    k_morton<uint64_t, 8, kSplitFirstPass, true><<<blocks, kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], hist, vparts, nvparts, tparts);
+  else if (mem->opts.mortonBits == 63)
+    k_morton<uint64_t, 8, kSplitFirstPass, false><<<blocks, kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], hist, nullptr, 0u, nullptr);
+  else if (check)
+    k_morton<uint32_t, 4, 0, true><<<blocks, kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], hist, vparts, nvparts, tparts);
   else
-    k_morton<uint32_t, 4, 0><<<mortonBlocks(n), kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, (const BuildParams *)mem->params, (uint32_t *)mem->keys[0], hist);
-  return checkLaunch("k_morton");
+    k_morton<uint32_t, 4, 0, false><<<blocks, kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], hist, nullptr, 0u, nullptr);
+  if (!checkLaunch("k_morton")) return false;
+  if (!check) return true;
+  timing().record("k_morton");
+  const uint32_t fixBlocks = std::min<uint32_t>(blocks, (uint32_t)numCUs());
+  if (mem->opts.mortonBits == 63)
+    k_mortonFix<uint64_t, 8, kSplitFirstPass><<<fixBlocks, kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], hist, tparts, blocks);
+  else
+    k_mortonFix<uint32_t, 4, 0><<<fixBlocks, kMortonBlock, 0, stream()>>>(
+        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], hist, tparts, blocks);
+  if (!checkLaunch("k_mortonFix")) return false;
+  timing().record("k_mortonFix");
+  return true;
 }
 
 bool runSort(bvht_LBVH_WorkingMemory *mem) {
@@ -437,12 +675,23 @@ bool buildAll(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAM
   Timing &tm = timing();
   tm.start();
   tm.record("begin");
-  k_sceneBox<<<in->boxBlocks, kBoxBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces, (BuildParams *)mem->params,
-                                                     in->partials, (uint32_t *)mem->sortScratch, sortZeroWords(in));
-  if (!checkLaunch("k_sceneBox")) return false;
-  tm.record("k_sceneBox");
-  if (!launchMorton(mem, mesh)) return false;
-  tm.record("k_morton");
+  // the vertex box (one streaming pass) when the keys come from k_morton,
+  // which checks it; the MSD key pass takes the triangles' box
+  const bool vertexBox = !in->useMsd;
+  if (vertexBox) {
+    k_vertBox<<<vertBlocks(mesh->numVert), kBoxBlock, 0, stream()>>>(mesh->vert, mesh->numVert, in->partials,
+                                                                    (uint32_t *)mem->sortScratch, sortZeroWords(in));
+    if (!checkLaunch("k_vertBox")) return false;
+    tm.record("k_vertBox");
+  } else {
+    k_sceneBox<<<in->boxBlocks, kBoxBlock, 0, stream()>>>(mesh->vert, mesh->faces, mesh->numFaces,
+                                                       (BuildParams *)mem->params, in->partials,
+                                                       (uint32_t *)mem->sortScratch, sortZeroWords(in));
+    if (!checkLaunch("k_sceneBox")) return false;
+    tm.record("k_sceneBox");
+  }
+  if (!launchMorton(mem, mesh, vertexBox)) return false;
+  if (!vertexBox) tm.record("k_morton");
   if (!runSort(mem)) return false;
   return runTree(mem, mesh, bvh);
 }
@@ -608,7 +857,7 @@ bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *me
       !devAlloc(&m.values[0], n, "LBVH values") || !devAlloc(&m.values[1], n, "LBVH values") ||
       !devMalloc(&m.sortScratch, m.sortScratchBytes, "LBVH sort scratch") ||
       !devMalloc(&m.params, sizeof(BuildParams), "LBVH params") ||
-      !devAlloc(&in->partials, (size_t)kBoxBlocksMax * 6, "LBVH partials") ||
+      !devAlloc(&in->partials, (size_t)kBoxBlocksMax * 12, "LBVH partials") ||
       !devAlloc(&m.atomicLocks, m.numLocks, "LBVH locks")) {
     LBVH_freeWorkingMemory(&m);
     return m;
@@ -699,7 +948,7 @@ void LBVH_calcMortonCodes(bvht_LBVH_WorkingMemory *mem, bvht_AABB sceneAABB) {
              "LBVH_calcMortonCodes params"))
     return;
   if (!fillU32((uint32_t *)mem->sortScratch, 0u, sortZeroWords(in))) return;
-  launchMorton(mem, &in->mesh);
+  launchMorton(mem, &in->mesh, false);
 }
 
 void LBVH_sortMortonCodes(bvht_LBVH_WorkingMemory *mem) {
@@ -840,7 +1089,7 @@ size_t carve(uint8_t *base, uint32_t n, int keyBytes, bvht_LBVH_WorkingMemory &m
   m.values[1] = reinterpret_cast<uint32_t *>(take(4ull * n));
   m.sortScratch = take(m.sortScratchBytes);
   m.params = take(sizeof(BuildParams));
-  in.partials = reinterpret_cast<float *>(take(sizeof(float) * kBoxBlocksMax * 6));
+  in.partials = reinterpret_cast<float *>(take(sizeof(float) * kBoxBlocksMax * 12));
   return off;
 }
 

@@ -10,7 +10,7 @@ struct BuildParams {
   float sceneMax[3];
   float offset[3];  // 0 - sceneMin          (LBVH_CUDA.cu:172)
   float scale[3];   // 1 / (sceneMax + offset) (LBVH_CUDA.cu:178-180)
-  uint32_t done;    // finished blocks of the scene-box reduction
+  uint32_t done;    // finished blocks of the scene-box reduction (staged calls)
   uint32_t pad[3];
 };
 

@@ -85,12 +85,26 @@ BVHT_HD bvht_AABB triangleBox(const bvht_vec3 &v1, const bvht_vec3 &v2, const bv
   return b;
 }
 
+// x / 3.0f, correctly rounded, in three operations instead of a general
+// division: q0 = x * RN(1/3), the remainder r = x - 3 q0 exactly (one FMA),
+// q0 + r * RN(1/3) rounded once (Markstein's correction).  An exact
+// quotient (r = 0: keeps the sign of a zero) or an infinite x (r is NaN)
+// returns q0.  Checked against IEEE division for every f32 bit pattern
+// (equal except for NaN payloads; tests/div3_check.c).
+BVHT_HD float div3(float x) {
+  const float third = 0.3333333432674408f;  // RN(1/3)
+  const float q0 = x * third;
+  const float r = __builtin_fmaf(-q0, 3.0f, x);
+  const float q1 = __builtin_fmaf(r, third, q0);
+  return (r == 0.0f || r != r) ? q0 : q1;
+}
+
 // Centroid (v1 + v2 + v3) / 3.0f -- lib/builder/LBVH_CUDA.cu:92.
 BVHT_HD bvht_vec3 centroid(const bvht_vec3 &v1, const bvht_vec3 &v2, const bvht_vec3 &v3) {
   bvht_vec3 c;
-  c.x = (v1.x + v2.x + v3.x) / 3.0f;
-  c.y = (v1.y + v2.y + v3.y) / 3.0f;
-  c.z = (v1.z + v2.z + v3.z) / 3.0f;
+  c.x = div3(v1.x + v2.x + v3.x);
+  c.y = div3(v1.y + v2.y + v3.y);
+  c.z = div3(v1.z + v2.z + v3.z);
   return c;
 }
 

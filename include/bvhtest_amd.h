@@ -240,7 +240,11 @@ void LBVH_freeWorkingMemory(bvht_LBVH_WorkingMemory *mem);
 /* LBVH_doCUDASyc -- lib/builder/LBVH_CUDA.cu:490 (stream sync). */
 void LBVH_doCUDASyc(void);
 /* Whole build on the stream without host round trips (extension: the scene
- * box stays on the device).  Same result as the five staged calls above. */
+ * box stays on the device).  Same result as the five staged calls above.
+ * The scene box comes from one pass over all mesh->numVert vertices (which
+ * must all be readable), checked against the triangles' box by the key
+ * pass, which repairs the keys when unreferenced vertices lie outside it
+ * (digitBits 16 uses the triangles' box directly). */
 bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh);
 /* Error bits of the last sort (extension, synchronises): 0 = the sort ran
  * to completion.  MSD sort: 1 = histogram does not match the keys, 2 = a

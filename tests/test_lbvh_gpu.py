@@ -143,6 +143,59 @@ def test_full_size_63bit_scene(bvh, orc):
         tree.free()
 
 
+class ShiftedMesh:
+    """A MeshRaw whose vertex array starts 4 bytes into its allocation (not
+    16-byte aligned)."""
+
+    def __init__(self, bvh, v, f):
+        import ctypes
+        v = np.ascontiguousarray(v, np.float32)
+        f = np.ascontiguousarray(f, np.uint32)
+        self.vbuf = bvh.DeviceBuffer(v.nbytes + 16)
+        self.fbuf = bvh.DeviceBuffer(f.nbytes).upload(f)
+        base = self.vbuf.ptr.value + 4
+        assert bvh.lib().runMemcpy(ctypes.c_void_p(base), v.ctypes.data, v.nbytes, 1)
+        self.raw = bvh.MeshRaw()
+        self.raw.vert, self.raw.faces = base, self.fbuf.ptr.value
+        self.raw.numVert, self.raw.numFaces = len(v), len(f)
+        self.num_faces, self.num_vert = len(f), len(v)
+
+
+def stray_vertex_cases(bvh):
+    """The full build takes its scene box from every vertex and checks it
+    against the triangles' box (the reference's), recomputing the keys when
+    they differ.  Unreferenced vertices outside the triangles' box take that
+    repair; inside it, or none at all, they do not."""
+    v, f = bvh.scene_heightfield(64, 32, 5, 0.2)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    out = {
+        "outside_max": (np.concatenate([v, [[5.0, 3.0, 7.0]]]).astype(np.float32), f),
+        "outside_min": (np.concatenate([[[-2.0, -1.0, -3.0]], v]).astype(np.float32), f + 1),
+        "inside": (np.concatenate([v, [(lo + hi) / 2]]).astype(np.float32), f),
+        "many_unreferenced": (np.concatenate([v, np.random.default_rng(3).random((1001, 3), dtype=np.float32) * 9 - 4]
+                                             ).astype(np.float32), f),
+    }
+    return out
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("name", ["outside_max", "outside_min", "inside", "many_unreferenced", "unaligned"])
+def test_vertex_box_check_and_repair(bvh, orc, name, bits):
+    cases = stray_vertex_cases(bvh)
+    v, f = cases["outside_max" if name == "unaligned" else name]
+    print(f"case {name} n={len(f)} V={len(v)} bits={bits}", flush=True)
+    keys, sf, expect, _ = orc.lbvh(v, f, bits)
+    mesh = ShiftedMesh(bvh, v, f) if name == "unaligned" else bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, bits, digit_bits=8)
+    tree = b.allocate_bvh()
+    for _ in range(2):  # the repair flag of one build does not leak into the next
+        b.build(tree)
+        assert b.sort_status() == 0
+        assert np.array_equal(b.sorted_faces(), sf)
+        assert nodes_equal(tree.download(), expect)
+    tree.free()
+
+
 def test_errors_are_reported(bvh):
     L = bvh.lib()
     v, f = soup(10)
