@@ -603,17 +603,111 @@ __global__ __launch_bounds__(256) void k_commitFirst(PatchRec *__restrict__ recs
 // Rounds 2.. over the (few) undecided patches, in one workgroup.  At most
 // `num + 1` rounds are needed; patches still undecided after that bound
 // (never, unless memory ordering is broken) are dropped and counted.
+//
+// Up to kTailBlock patches (one per thread), the rounds run in LDS and
+// registers: the nodes of the undecided patches go into an LDS hash table
+// holding, per node, whether a winner took it (round 1's from its global
+// lock, read once) and the lowest rank reserving it this round, and each
+// thread keeps its patch's table slots in registers.  A round is then LDS
+// atomics and barriers instead of global atomics and loads.  The decisions
+// are those of the global rounds: in round r a patch reserves every node no
+// winner took, wins if it holds them all, loses if a winner took one, and
+// waits otherwise.  Global locks keep their round-1 values (k_apply
+// releases them all).
 constexpr int kTailBlock = 1024;
+constexpr uint32_t kTailSlotsLog = 14, kTailSlots = 1u << kTailSlotsLog;  // LDS hash slots (load <= 0.63)
+constexpr uint32_t kTailEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t tailHash(uint32_t n) { return (n * 0x9E3779B1u) >> (32 - kTailSlotsLog); }
+
+// Slot of node n, inserted if absent (the table never fills).
+__device__ __forceinline__ uint32_t tailInsert(uint32_t *keys, uint32_t n) {
+  uint32_t h = tailHash(n);
+  for (uint32_t probe = 0; probe < kTailSlots; ++probe) {
+    const uint32_t prev = atomicCAS(keys + h, kTailEmpty, n);
+    if (prev == kTailEmpty || prev == n) return h;
+    h = (h + 1) & (kTailSlots - 1);
+  }
+  return kTailEmpty;  // unreachable
+}
+
 __global__ __launch_bounds__(kTailBlock) void k_resolveTail(PatchRec *__restrict__ recs, uint64_t *locks,
                                                            uint32_t *__restrict__ marks,
                                                            uint32_t *__restrict__ skipped, uint32_t *__restrict__ stats,
                                                            uint32_t *listA, uint32_t *listB,
                                                            const uint32_t *__restrict__ numPending) {
+  __shared__ uint32_t s_key[kTailSlots];   // node id, kTailEmpty
+  __shared__ uint32_t s_min[kTailSlots];   // lowest reserving rank this round
+  __shared__ uint8_t s_taken[kTailSlots];  // taken by a winner
   __shared__ uint32_t s_cnt;
-  uint32_t *cur = listA, *nxt = listB;
   uint32_t num = *numPending;
   const uint32_t maxRound = num + 2;
   uint32_t round = 2;
+  if (num <= (uint32_t)kTailBlock) {
+    for (uint32_t i = threadIdx.x; i < kTailSlots; i += kTailBlock) s_key[i] = kTailEmpty;
+    __syncthreads();
+    const bool mine = threadIdx.x < num;
+    const uint32_t k = mine ? listA[threadIdx.x] : 0u;
+    uint32_t sl[kNNode];
+    for (int j = 0; j < kNNode; ++j) {
+      const uint32_t n = mine ? recs[k].node[j] : NONE;
+      sl[j] = n == NONE ? kTailEmpty : tailInsert(s_key, n);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTailSlots; i += kTailBlock) {
+      s_min[i] = kTailEmpty;
+      const uint32_t n = s_key[i];
+      s_taken[i] = n != kTailEmpty &&
+                   (__hip_atomic_load(locks + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) == 0;
+    }
+    bool open = mine;  // undecided
+    __syncthreads();
+    for (; num > 0 && round <= maxRound; ++round) {
+      if (open)  // reserve
+        for (int j = 0; j < kNNode; ++j)
+          if (sl[j] != kTailEmpty && !s_taken[sl[j]]) atomicMin(s_min + sl[j], k);
+      if (threadIdx.x == 0) s_cnt = 0;
+      __syncthreads();
+      if (open) {  // decide
+        bool holdAll = true, lost = false;
+        for (int j = 0; j < kNNode; ++j) {
+          if (sl[j] == kTailEmpty) continue;
+          if (s_taken[sl[j]]) lost = true;
+          if (s_min[sl[j]] != k) holdAll = false;
+        }
+        PatchRec *r = recs + k;
+        if (lost) {
+          r->ok = REC_LOST;
+          skipped[k] += 1;
+          atomicAdd(stats + ST_CONFLICT, 1u);
+          open = false;
+        } else if (holdAll) {
+          for (int j = 0; j < kNNode; ++j)
+            if (sl[j] != kTailEmpty) s_taken[sl[j]] = 1;
+          r->ok = REC_WON;
+          marks[r->moved[0]] = k;  // moved roots are patch nodes: one applied patch each
+          marks[r->moved[1]] = k;
+          open = false;
+        } else {
+          atomicAdd(&s_cnt, 1u);
+        }
+      }
+      __syncthreads();
+      num = s_cnt;
+      for (uint32_t i = threadIdx.x; i < kTailSlots; i += kTailBlock) s_min[i] = kTailEmpty;
+      __syncthreads();
+    }
+    if (open) {
+      recs[k].ok = REC_LOST;
+      skipped[k] += 1;
+      atomicAdd(stats + ST_CONFLICT, 1u);
+      atomicAdd(stats + ST_UNRESOLVED, 1u);
+    }
+    return;
+  }
+  // more undecided patches than the LDS table takes: the rounds on the
+  // global locks
+  uint32_t *cur = listA, *nxt = listB;
   for (; num > 0 && round <= maxRound; ++round) {
     for (uint32_t i = threadIdx.x; i < num; i += kTailBlock) reserve(recs + cur[i], cur[i], round, locks);
     if (threadIdx.x == 0) s_cnt = 0;
