@@ -219,8 +219,8 @@ __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict_
 //    the pass's digit (a digit never straddles the words).
 //  - The identity values of a first pass are made when they are staged.
 //  - The tile's global offsets are the sums of every earlier tile's
-//    published per-digit counts, loaded by all 1024 threads at once (four
-//    per digit, 16 loads in flight each) instead of a chained look-back.
+//    published per-digit counts, loaded by all 1024 threads at once (16-byte
+//    loads of four digits, 8 in flight each) instead of a chained look-back.
 //    Tile ids come from `counter`, so every tile a workgroup waits for has
 //    started; the wait for a count is bounded and raises the error flag.
 template <typename K, bool IDENTITY>
@@ -234,8 +234,6 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   constexpr int WAVES = kWideBlock / 64;
   constexpr int ITEMS = kWideItems;
   constexpr uint32_t TILE = kWideTile;
-  constexpr uint32_t GROUPS = kWideBlock / kSortRadix;  // threads per digit in the prefix sums
-  constexpr uint32_t BATCH = 16;                        // predecessor counts per thread per round trip
   constexpr bool WIDE = sizeof(K) == 8;
   __shared__ uint32_t s_wave[WAVES][kSortRadix];  // per-wave digit counters, then prefix partials
   __shared__ uint32_t s_bstart[kSortRadix];
@@ -331,41 +329,50 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   }
   __syncthreads();  // s_wave is free from here: it holds the prefix partials
 
-  // offsets: the counts of every earlier tile, digit d summed by the GROUPS
-  // threads d, d + 256, ... over predecessors g, g + GROUPS, ...
+  // offsets: the counts of every earlier tile.  Thread (g, q) sums digits
+  // 4q .. 4q + 3 over predecessors g, g + 16, ... with 16-byte coherent
+  // loads, 8 in flight (two round trips for 243 predecessors); the 16
+  // partial sums of a digit are then added by its owner.
   {
-    const uint32_t d = tid & (kSortRadix - 1), g = tid >> kSortRadixBits;
-    uint32_t sum = 0, spins = 0;
-    for (uint32_t j0 = g; j0 < tile; j0 += GROUPS * BATCH) {
-      uint32_t sv[BATCH];
+    constexpr uint32_t QUADS = kSortRadix / 4, QG = kWideBlock / QUADS, QB = 8;
+    const uint32_t q = tid % QUADS, g = tid / QUADS;
+    const BufferRsrc st = bufferRsrc(status, tile * kSortRadix * 4u);  // the earlier tiles' words
+    uint32_t sum[4] = {0u, 0u, 0u, 0u}, spins = 0;
+    for (uint32_t j0 = g; j0 < tile; j0 += QG * QB) {
+      BufferU32x4 sv[QB];
 #pragma unroll
-      for (uint32_t b = 0; b < BATCH; ++b) {
-        const uint32_t j = j0 + b * GROUPS;
-        sv[b] = j < tile ? loadRelaxedAgent(status + (size_t)j * kSortRadix + d) : kFlagAgg;
+      for (uint32_t b = 0; b < QB; ++b) {
+        const uint32_t j = j0 + b * QG;
+        sv[b] = j < tile ? bufferLoadU32x4Coherent(st, (j * kSortRadix + 4 * q) * 4u)
+                         : BufferU32x4{kFlagAgg, kFlagAgg, kFlagAgg, kFlagAgg};
       }
 #pragma unroll
-      for (uint32_t b = 0; b < BATCH; ++b) {
-        const uint32_t j = j0 + b * GROUPS;
-        uint32_t v = sv[b];
-        while ((v & ~kCountMask) == 0) {  // not yet published (j < tile: tile 0 never waits)
+      for (uint32_t b = 0; b < QB; ++b) {
+        const uint32_t j = j0 + b * QG;
+        BufferU32x4 v = sv[b];
+        // not yet published (j < tile: tile 0 never waits); the wait is bounded
+        while (((v[0] & ~kCountMask) == 0 || (v[1] & ~kCountMask) == 0 || (v[2] & ~kCountMask) == 0 ||
+                (v[3] & ~kCountMask) == 0)) {
           if (++spins > (1u << 24)) {
             atomicOr(counter + 8, 1u);
-            v = kFlagAgg;
+            v = BufferU32x4{kFlagAgg, kFlagAgg, kFlagAgg, kFlagAgg};
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          v = loadRelaxedAgent(status + (size_t)j * kSortRadix + d);
+          v = bufferLoadU32x4Coherent(st, (j * kSortRadix + 4 * q) * 4u);
         }
-        sum += v & kCountMask;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sum[c] += v[c] & kCountMask;
       }
     }
-    s_wave[g][d] = sum;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_wave[g][4 * q + c] = sum[c];
   }
   __syncthreads();
   if (tid < kSortRadix) {
     uint32_t excl = 0;
 #pragma unroll
-    for (uint32_t g = 0; g < GROUPS; ++g) excl += s_wave[g][tid];
+    for (uint32_t g = 0; g < WAVES; ++g) excl += s_wave[g][tid];
     s_gstart[tid] = hbase + excl;
   }
   __syncthreads();
@@ -439,6 +446,10 @@ bool sortHistogram(const SortPlan &p, const void *keys, uint32_t *scratch) {
 int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityValues, bool keepKeys, uint32_t *scratch,
             void (*afterPass)(int), int firstPass) {
   if (p.n == 0) return 0;
+  if (((uintptr_t)(scratch + p.statusOff) & 15) != 0) {  // the wide pass reads statuses 16 bytes at a time
+    setError("sort scratch must be 16-byte aligned");
+    return -1;
+  }
   int cur = 0;
   for (int pass = firstPass; pass < p.passes; ++pass) {
     const uint32_t *hist = scratch + p.histOff + (size_t)pass * kSortRadix;
