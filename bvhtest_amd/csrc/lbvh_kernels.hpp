@@ -17,15 +17,51 @@ struct SegDesc {
   uint32_t vertBase;  // first vertex of the mesh
 };
 
-__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+// The three 16-byte words of a 48-byte node.
+struct NodeWords {
+  uint4 w[3];
+};
+__device__ __forceinline__ NodeWords packNode(const bvht_AABB &b, uint32_t parent, uint32_t left, uint32_t right,
+                                              uint32_t isLeaf, uint32_t isLeft, uint32_t level, float sa) {
+  NodeWords o;
+  o.w[0] = make_uint4(__float_as_uint(b.minMax[0].x), __float_as_uint(b.minMax[0].y), __float_as_uint(b.minMax[0].z),
+                      __float_as_uint(b.minMax[1].x));
+  o.w[1] = make_uint4(__float_as_uint(b.minMax[1].y), __float_as_uint(b.minMax[1].z), parent, left);
+  o.w[2] = make_uint4(right, isLeaf | (isLeft << 16), level, __float_as_uint(sa));
+  return o;
+}
+
+// The 64 lanes of a wave write the nodes dst[0..63] (lane l's node, if
+// `write`) through an LDS staging buffer of 64 nodes, so that every store
+// instruction covers 1 KB of consecutive bytes instead of 16 bytes every 48
+// (three instructions per node, each touching 24 lines in part).  All
+// threads of the workgroup call it (barriers: lanes read other lanes'
+// words, and the next call reuses the buffer).
+__device__ __forceinline__ void storeNodesStaged(uint4 *stage, uint8_t *stageOk, bvht_BVHNode *dst, const NodeWords &nw,
+                                                 bool write) {
+  const uint32_t lane = threadIdx.x & 63;
+  stage[3 * lane] = nw.w[0];
+  stage[3 * lane + 1] = nw.w[1];
+  stage[3 * lane + 2] = nw.w[2];
+  stageOk[lane] = write ? 1 : 0;
+  __syncthreads();
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t c = lane + 64 * q;
+    if (stageOk[c / 3]) d[c] = stage[c];
+  }
+  __syncthreads();
+}
 
 // Writes one full 48-byte node with three 16-byte stores.
 __device__ __forceinline__ void storeNode(bvht_BVHNode *dst, const bvht_AABB &b, uint32_t parent, uint32_t left,
                                           uint32_t right, uint32_t isLeaf, uint32_t isLeft, uint32_t level, float sa) {
+  const NodeWords nw = packNode(b, parent, left, right, isLeaf, isLeft, level, sa);
   uint4 *d = reinterpret_cast<uint4 *>(dst);
-  d[0] = make_uint4(f2u(b.minMax[0].x), f2u(b.minMax[0].y), f2u(b.minMax[0].z), f2u(b.minMax[1].x));
-  d[1] = make_uint4(f2u(b.minMax[1].y), f2u(b.minMax[1].z), parent, left);
-  d[2] = make_uint4(right, isLeaf | (isLeft << 16), level, f2u(sa));
+  d[0] = nw.w[0];
+  d[1] = nw.w[1];
+  d[2] = nw.w[2];
 }
 
 // Parent index and isLeft flag of the node covering [l, r] (closed form of
@@ -92,6 +128,17 @@ __device__ __forceinline__ void boxStep(float b[6], uint32_t k, uint32_t half, u
   }
 }
 
+// Node words and index of the inner node covering [l, r].
+__device__ __forceinline__ NodeWords innerWords(uint32_t l, uint32_t r, uint32_t N1, const float b[6]) {
+  bvht_AABB box;
+  box.minMax[0] = {b[0], b[1], b[2]};
+  box.minMax[1] = {b[3], b[4], b[5]};
+  const uint32_t m = rangeSplit(l, r);
+  uint32_t parent, isLeft;
+  parentOf(l, r, N1, parent, isLeft);
+  return packNode(box, parent, rangeNodeIndex(l, m - 1, N1), rangeNodeIndex(m, r, N1), 0u, isLeft, 0u, surfaceArea(box));
+}
+
 __device__ __forceinline__ void storeInner(bvht_BVHNode *nodes, uint32_t l, uint32_t r, uint32_t N1, const float b[6]) {
   bvht_AABB box;
   box.minMax[0] = {b[0], b[1], b[2]};
@@ -111,8 +158,11 @@ __device__ __forceinline__ void storeInner(bvht_BVHNode *nodes, uint32_t l, uint
 // node in the tile lies in the tile).  Boxes of aligned blocks of up to 64
 // positions come from butterfly shuffles inside the wave; the few larger
 // blocks of the tile are reduced in LDS from the 64-position blocks.
+// Six workgroups per CU (at most 80 VGPRs) measure faster than four (the
+// compiler's 105 VGPRs without the bound) and than eight (64 VGPRs, with
+// spills).
 template <int TLOG, bool BATCH>
-__global__ __launch_bounds__(kBuildBlock) void k_buildLevel0(bvht_BVHNode *__restrict__ nodesAll,
+__global__ __launch_bounds__(kBuildBlock, 6) void k_buildLevel0(bvht_BVHNode *__restrict__ nodesAll,
                                                              const uint32_t *__restrict__ sorted,
                                                              const bvht_Triangle *__restrict__ faces,
                                                              const bvht_vec3 *__restrict__ vertAll, SegDesc single,
@@ -161,18 +211,27 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel0(bvht_BVHNode *__res
     vb[i] = vert[t[i].v2];
     vc[i] = vert[t[i].v3];
   }
-  // leaves first: the gathered vertices are dead before the reductions start
+  // leaves first: the gathered vertices are dead before the reductions start.
+  // A wave's 64 leaves (positions k - lane .. k - lane + 63) are written
+  // through the LDS staging buffer.
+  __shared__ uint4 s_stage[kBuildBlock / 64][64 * 3];
+  __shared__ uint8_t s_stageOk[kBuildBlock / 64][64];
+  uint4 *stage = s_stage[threadIdx.x >> 6];
+  uint8_t *stageOk = s_stageOk[threadIdx.x >> 6];
+  const uint32_t lane = threadIdx.x & 63;
   float leaf[R][6];
+  bvht_AABB lbs[R];  // every leaf box first: the vertices die before the barriers below
+#pragma unroll
+  for (int i = 0; i < R; ++i) lbs[i] = triangleBox(va[i], vb[i], vc[i]);
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const uint32_t k = threadIdx.x + i * kBuildBlock, p = u0 + k;
-    const bvht_AABB lb = triangleBox(va[i], vb[i], vc[i]);
-    if (k < cnt) {
-      uint32_t parent, isLeft;
-      parentOf(p, p, N1, parent, isLeft);
-      // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
-      storeNode(nodes + N1 + p, lb, parent, f[i], 1u, 1u, isLeft, 255u, surfaceArea(lb));
-    }
+    const bvht_AABB lb = lbs[i];
+    uint32_t parent = 0, isLeft = 0;
+    if (k < cnt) parentOf(p, p, N1, parent, isLeft);
+    // kGenLeafNodes (LBVH_CUDA.cu:273-292): left = face, right = 1, level = UINT8_MAX
+    storeNodesStaged(stage, stageOk, nodes + N1 + (p - lane), packNode(lb, parent, f[i], 1u, 1u, isLeft, 255u, surfaceArea(lb)),
+                     k < cnt);
     leaf[i][0] = lb.minMax[0].x;
     leaf[i][1] = lb.minMax[0].y;
     leaf[i][2] = lb.minMax[0].z;
@@ -203,7 +262,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevel0(bvht_BVHNode *__res
 #pragma unroll
       for (int c = 0; c < 6; ++c) s_box[k >> 6][c] = b[c];
     }
-    if (jn >= 1 && jn <= 6) storeInner(nodes, l, r, N1, pick);
+    // the inner node with index p is the lane's: staged like the leaves
+    storeNodesStaged(stage, stageOk, nodes + (p - lane), innerWords(l, r, N1, pick), jn >= 1 && jn <= 6);
   }
   __syncthreads();
   // levels 7..TLOG: block a of level j merges blocks 2a and 2a + 1 of level j - 1
