@@ -14,6 +14,8 @@
 //                mesh, the look-back stops at the mesh's first tile, and
 //                digit offsets start at the mesh's prefix; u32 keys, 8- or
 //                10-bit digits (4 or 3 passes, chosen by the autotuner);
+//   k_segOnesweepWide the 8-bit pass over tiles of 1024 threads x 11 keys,
+//                look-back in rounds of 16 predecessors of the mesh;
 //   k_buildLevel0 / k_buildLevelUp the tile hierarchy per mesh (tiles
 //                never cross meshes).
 #include "lbvh_kernels.hpp"
@@ -72,6 +74,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_segBox(const bvht_vec3 *__restr
 
 constexpr int kSegItems = 15;                               // keys per thread of a sort tile
 constexpr uint32_t kSegTile = kSortBlock * kSegItems;       // pairs per sort tile
+constexpr int kSegWideItems = 11;                           // keys per thread of a wide sort tile
+constexpr uint32_t kSegWideTile = kWideBlock * kSegWideItems;
 constexpr int kChunkLog = 15;                               // triangles per Morton workgroup
 constexpr int kSegMaxHist = 3 * 1024;                       // passes x radix, both digit widths
 constexpr uint32_t kSegFlagAgg = 1u << 30, kSegFlagInc = 2u << 30, kSegCountMask = (1u << 30) - 1;
@@ -172,8 +176,8 @@ __global__ __launch_bounds__(kSortBlock) void k_segOnesweep(const uint32_t *__re
                                                             const uint2 *__restrict__ tileTab,
                                                             const SegDesc *__restrict__ segs, uint32_t shift,
                                                             const uint32_t *__restrict__ prefix, int passes,
-                                                            uint32_t *__restrict__ status, uint32_t *__restrict__ counter,
-                                                            uint32_t *__restrict__ err) {
+                                                            uint32_t *status, uint32_t *counter,
+                                                            uint32_t *err) {
   constexpr int R = 1 << RB, D = R / kSortBlock, WAVES = kSortBlock / 64, TILE = (int)kSegTile;
   __shared__ uint32_t s_wave[WAVES][R];
   __shared__ uint32_t s_bstart[R], s_gstart[R];
@@ -299,6 +303,187 @@ __global__ __launch_bounds__(kSortBlock) void k_segOnesweep(const uint32_t *__re
   }
 }
 
+// One pass of the segmented onesweep with 8-bit digits over wide tiles
+// (kSegWideTile keys; two workgroups per CU: 8 waves per SIMD, at most 64
+// VGPRs, 63 KB of LDS each), for batches whose key array
+// stays below 2^31 bytes (buffer accesses with 32-bit offsets; a load past
+// the end reads 0).  Same order as k_segOnesweep: stable by (mesh, digit).
+// Look-back in rounds of 16 predecessors of the same mesh: thread (g, q)
+// loads digits 4q .. 4q + 3 of the predecessor at distance r + g + 1 (one
+// 16-byte coherent load), waits only until that word is published
+// (aggregate or inclusive), and each digit's owner adds the counts back to
+// the nearest inclusive one.  No tile waits for another's look-back, so the
+// walk is one round unless a digit's nearest inclusive count lies more than
+// 16 tiles back; a mesh's first tile publishes an inclusive count at once.
+__global__ __launch_bounds__(kWideBlock, 8) void k_segOnesweepWide(
+    const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn, uint32_t *__restrict__ keysOut,
+    uint32_t *__restrict__ valsOut, const uint2 *__restrict__ tileTab, const SegDesc *__restrict__ segs,
+    uint32_t total, uint32_t shift, const uint32_t *__restrict__ prefix, int passes, uint32_t *status,
+    uint32_t *counter, uint32_t *err) {
+  constexpr int WAVES = kWideBlock / 64, ITEMS = kSegWideItems, R = kSortRadix;
+  constexpr uint32_t TILE = kSegWideTile, QUADS = R / 4, NONE = 0x80000000u;
+  static_assert(kWideBlock / QUADS == WAVES, "one look-back group per wave");
+  __shared__ uint32_t s_wave[WAVES][R];  // per-wave digit counters, then look-back words
+  __shared__ uint32_t s_bstart[R], s_gstart[R];
+  __shared__ uint32_t s_scan[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_buf[TILE];  // the staged keys, then the values
+
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  for (uint32_t i = tid; i < (uint32_t)WAVES * R; i += kWideBlock) (&s_wave[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t g = s_tile;
+  const uint2 tt = tileTab[g];
+  const SegDesc sd = segs[tt.x];
+  const uint32_t lt = tt.y, segStart = sd.faceBase, segN = sd.n;
+  const uint32_t first = lt * TILE + wave * 64 * ITEMS + lane;  // mesh-local index of item 0
+  const BufferRsrc keys = bufferRsrc(keysIn, total * 4u);
+  uint32_t kw[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t idx = first + i * 64;
+    kw[i] = bufferLoadU32(keys, idx < segN ? (segStart + idx) * 4u : NONE);
+  }
+  uint32_t *cnt = &s_wave[wave][0];
+  const uint64_t lt64 = __lanemask_lt();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool valid = first + i * 64 < segN;
+    const uint32_t d = (kw[i] >> shift) & (R - 1);
+    const uint64_t peers = matchDigit<kSortRadixBits>(d, __ballot(valid));
+    const uint32_t lower = (uint32_t)__popcll(peers & lt64);
+    const uint32_t c = __hip_atomic_load(cnt + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    rank[i] = c + lower;
+    if (valid && lower == 0)
+      __hip_atomic_store(cnt + d, c + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+
+  // digit owners (tid < R): per-wave exclusive offsets and the tile's count,
+  // published at once (inclusive for the mesh's first tile)
+  uint32_t count = 0;
+  if (tid < R) {
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_wave[w][tid];
+      s_wave[w][tid] = count;
+      count += c;
+    }
+    storeRelaxedAgent(status + (size_t)g * R + tid, (lt == 0 ? kSegFlagInc : kSegFlagAgg) | count);
+  }
+  const uint32_t bstart = blockExclusiveScan<kWideBlock>(count, s_scan);
+  if (tid < R) s_bstart[tid] = bstart;
+  __syncthreads();
+
+  // keys to their tile positions (digits and item indices recomputed)
+  uint32_t first2 = first;
+  recomputeAfter(first2);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) recomputeAfter(kw[i]);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if (first2 + i * 64 < segN) {
+      const uint32_t d = (kw[i] >> shift) & (R - 1);
+      rank[i] += s_bstart[d] + s_wave[wave][d];
+      s_buf[rank[i]] = kw[i];
+    }
+  }
+  __syncthreads();  // s_wave is free from here: it holds the look-back words
+
+  uint32_t excl = 0;
+  if (lt > 0) {  // block-uniform
+    const uint32_t q = tid % QUADS, grp = tid / QUADS;
+    bool done = false;
+    uint32_t spins = 0;
+    for (uint32_t r0 = 0;; r0 += WAVES) {
+      const uint32_t dist = r0 + grp + 1;
+      BufferU32x4 v = BufferU32x4{kSegFlagInc, kSegFlagInc, kSegFlagInc, kSegFlagInc};  // before the mesh: 0
+      if (dist <= lt) {
+        const BufferRsrc st = bufferRsrc(status + (size_t)(g - dist) * R, R * 4u);
+        v = bufferLoadU32x4Coherent(st, 16u * q);
+        // a predecessor publishes right after its ranking; the wait is bounded
+        while ((v[0] & ~kSegCountMask) == 0 || (v[1] & ~kSegCountMask) == 0 || (v[2] & ~kSegCountMask) == 0 ||
+               (v[3] & ~kSegCountMask) == 0) {
+          if (++spins > (1u << 24)) {
+            atomicOr(err, 1u);
+            v = BufferU32x4{kSegFlagInc, kSegFlagInc, kSegFlagInc, kSegFlagInc};
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          memoryChanged();
+          v = bufferLoadU32x4Coherent(st, 16u * q);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s_wave[grp][4 * q + c] = v[c];
+      __syncthreads();
+      if (tid < R && !done) {
+        for (int w = 0; w < WAVES; ++w) {
+          const uint32_t x = s_wave[w][tid];
+          excl += x & kSegCountMask;
+          if ((x & ~kSegCountMask) == kSegFlagInc) {
+            done = true;
+            break;
+          }
+        }
+      }
+      // the barrier also keeps the next round's words from overwriting these
+      if (!__syncthreads_or(tid < R && !done)) break;
+    }
+  }
+  if (tid < R) {
+    if (lt > 0) storeRelaxedAgent(status + (size_t)g * R + tid, kSegFlagInc | (excl + count));
+    s_gstart[tid] = segStart + prefix[(size_t)tt.x * passes * R + tid] + excl;
+  }
+  __syncthreads();
+
+  // the destination of tile position j is s_gstart[d] + j - s_bstart[d];
+  // the digits of a thread's positions are kept four to a register
+  const uint32_t tileCount = min(TILE, segN - lt * TILE), segEnd = segStart + segN;
+  const BufferRsrc kout = bufferRsrc(keysOut, keysOut ? total * 4u : 0u);
+  const BufferRsrc vout = bufferRsrc(valsOut, total * 4u);
+  auto dest = [&](uint32_t j, uint32_t d) {
+    const uint32_t o = s_gstart[d] + (j - s_bstart[d]);
+    return j < tileCount && o < segEnd ? o * 4u : NONE;  // past the mesh only after a bounded-wait error
+  };
+  uint32_t dig[(ITEMS + 3) / 4] = {};
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = tid + k * kWideBlock;
+    const uint32_t kk = s_buf[j < tileCount ? j : 0];
+    const uint32_t d = (kk >> shift) & (R - 1);
+    dig[k / 4] |= d << (8 * (k % 4));
+    if (keysOut) bufferStore(kk, kout, dest(j, d));
+  }
+  // values: loaded only now (the other workgroup on the CU covers the
+  // latency), so that they are not live beside the ranks during the keys'
+  // look-back and scatter
+  uint32_t first3 = first;
+  recomputeAfter(first3);
+  uint32_t val[ITEMS];
+  if (valsIn) {
+    const BufferRsrc vals = bufferRsrc(valsIn, total * 4u);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t idx = first3 + i * 64;
+      val[i] = bufferLoadU32(vals, idx < segN ? (segStart + idx) * 4u : NONE);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) val[i] = segStart + first3 + i * 64;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (first3 + i * 64 < segN) s_buf[rank[i]] = val[i];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t j = tid + k * kWideBlock;
+    bufferStore(s_buf[j < tileCount ? j : 0], vout, dest(j, (dig[k / 4] >> (8 * (k % 4))) & (R - 1)));
+  }
+}
+
 struct BatchInternal {
   uint32_t M = 0, totalFaces = 0, totalNodes = 0, totalVert = 0;
   int tlog = 10;
@@ -314,6 +499,8 @@ struct BatchInternal {
   uint32_t numChunks = 0;
   uint2 *sortTiles = nullptr;  // (mesh, tile) of the sort passes
   uint32_t numSortTiles = 0;
+  uint2 *wideTiles = nullptr;  // (mesh, tile) of the wide 8-bit passes
+  uint32_t numWideTiles = 0;
   // sort scratch: [0, 16) pass counters + error word, then the mesh
   // histograms / prefixes (M x passes x radix), then the look-back status
   // (passes x sort tiles x radix); all of it is zeroed before every build
@@ -325,6 +512,9 @@ struct BatchInternal {
   const void *capVert = nullptr, *capFaces = nullptr, *capNodes = nullptr;
 };
 
+// Wide 8-bit passes apply while the key array stays below 2^31 bytes.
+bool useWideSegSort(const BatchInternal *in) { return in->radixBits == 8 && in->totalFaces < (1u << 29); }
+
 template <int RB>
 bool segSort(BatchInternal *in, int &result) {
   constexpr int R = 1 << RB, P = segPasses(RB);
@@ -335,6 +525,15 @@ bool segSort(BatchInternal *in, int &result) {
   int cur = 0;
   for (int pass = 0; pass < P; ++pass) {
     const bool last = pass == P - 1;
+    if (RB == 8 && useWideSegSort(in)) {
+      k_segOnesweepWide<<<in->numWideTiles, kWideBlock, 0, stream()>>>(
+          in->keys[cur], pass == 0 ? nullptr : in->vals[cur], last ? nullptr : in->keys[cur ^ 1], in->vals[cur ^ 1],
+          in->wideTiles, in->segs, in->totalFaces, (uint32_t)(pass * RB), hist + (size_t)pass * R, P,
+          status + (size_t)pass * in->numWideTiles * R, counters + pass, counters + 15);
+      if (!checkLaunch("k_segOnesweepWide")) return false;
+      cur ^= 1;
+      continue;
+    }
     k_segOnesweep<RB><<<in->numSortTiles, kSortBlock, 0, stream()>>>(
         in->keys[cur], pass == 0 ? nullptr : in->vals[cur], last ? nullptr : in->keys[cur ^ 1], in->vals[cur ^ 1],
         in->sortTiles, in->segs, (uint32_t)(pass * RB), hist + (size_t)pass * R, P,
@@ -350,7 +549,8 @@ template <int TLOG>
 bool batchStagesT(BatchInternal *in, const bvht_vec3 *vert, const bvht_Triangle *faces, bvht_BVHNode *nodes) {
   if (!fillU32(in->boxOrd, kOrdZero, 6ull * in->M)) return false;
   const int P = segPasses(in->radixBits), R = 1 << in->radixBits;
-  if (!fillU32(in->sortScratch, 0u, 16 + (size_t)in->M * P * R + (size_t)P * in->numSortTiles * R)) return false;
+  const uint32_t sortTiles = useWideSegSort(in) ? in->numWideTiles : in->numSortTiles;
+  if (!fillU32(in->sortScratch, 0u, 16 + (size_t)in->M * P * R + (size_t)P * sortTiles * R)) return false;
   k_segBox<TLOG><<<in->tileCounts[0], kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->tiles[0], in->boxOrd);
   if (!checkLaunch("k_segBox")) return false;
   int r = 0;
@@ -482,14 +682,16 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
     return m;
   }
   // Morton chunks and sort tiles: per mesh, never crossing one
-  std::vector<uint2> chunks, sortTiles;
+  std::vector<uint2> chunks, sortTiles, wideTiles;
   for (uint32_t i = 0; i < numMeshes; ++i) {
     const uint32_t n = faceCounts[i];
     for (uint32_t c = 0; (uint64_t)c << kChunkLog < n; ++c) chunks.push_back(make_uint2(i, c));
     for (uint32_t t = 0; (uint64_t)t * kSegTile < n; ++t) sortTiles.push_back(make_uint2(i, t));
+    for (uint32_t t = 0; (uint64_t)t * kSegWideTile < n; ++t) wideTiles.push_back(make_uint2(i, t));
   }
   in->numChunks = (uint32_t)chunks.size();
   in->numSortTiles = (uint32_t)sortTiles.size();
+  in->numWideTiles = (uint32_t)wideTiles.size();
   // scratch for the larger of the two digit widths (10 bits: 3 passes x 1024)
   in->histWords = (size_t)numMeshes * kSegMaxHist;
   in->statusWords = (size_t)3 * in->numSortTiles * 1024;
@@ -499,6 +701,9 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
       !devAlloc(&in->sortTiles, sortTiles.size(), "batch sort tiles") ||
       !check(hipMemcpy(in->sortTiles, sortTiles.data(), sizeof(uint2) * sortTiles.size(), hipMemcpyHostToDevice),
              "batch sort tiles") ||
+      !devAlloc(&in->wideTiles, wideTiles.size(), "batch wide sort tiles") ||
+      !check(hipMemcpy(in->wideTiles, wideTiles.data(), sizeof(uint2) * wideTiles.size(), hipMemcpyHostToDevice),
+             "batch wide sort tiles") ||
       !devAlloc(&in->segs, numMeshes, "batch segments") ||
       !check(hipMemcpy(in->segs, in->segsHost.data(), sizeof(SegDesc) * numMeshes, hipMemcpyHostToDevice),
              "batch segments") ||
@@ -582,6 +787,7 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem) {
     freeTiles(in);
     devFree(in->chunks);
     devFree(in->sortTiles);
+    devFree(in->wideTiles);
     devFree(in->segs);
     devFree(in->boxOrd);
     devFree(in->keys[0]);

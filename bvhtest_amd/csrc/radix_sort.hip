@@ -42,7 +42,7 @@ template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
                                                          K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                          uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
-                                                         uint32_t *__restrict__ status, uint32_t *__restrict__ counter) {
+                                                         uint32_t *status, uint32_t *counter) {
   constexpr int WAVES = kSortBlock / 64;
   constexpr int TILE = kSortBlock * ITEMS;
   static_assert(kSortBlock == kSortRadix, "one digit per thread");
@@ -229,8 +229,8 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
                                                              K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                              uint32_t n, uint32_t shift,
                                                              const uint32_t *__restrict__ hist,
-                                                             uint32_t *__restrict__ status,
-                                                             uint32_t *__restrict__ counter) {
+                                                             uint32_t *status,
+                                                             uint32_t *counter) {
   constexpr int WAVES = kWideBlock / 64;
   constexpr int ITEMS = kWideItems;
   constexpr uint32_t TILE = kWideTile;
@@ -359,6 +359,7 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
             break;
           }
           __builtin_amdgcn_s_sleep(1);
+          memoryChanged();
           v = bufferLoadU32x4Coherent(st, (j * kSortRadix + 4 * q) * 4u);
         }
 #pragma unroll

@@ -49,11 +49,17 @@ __device__ __forceinline__ uint32_t bufferLoadU32(BufferRsrc r, uint32_t off) {
 }
 // 16 bytes (16-byte aligned) with the cache policy of a relaxed agent-scope
 // atomic load (sc1: past the XCD's L2), for words other workgroups publish
-// with storeRelaxedAgent; each dword is read whole.
+// with storeRelaxedAgent; each dword is read whole.  To the optimiser this
+// is a plain read: a reload in a spin loop must follow memoryChanged(),
+// or it is merged with the first load and the loop never sees the word
+// change.
 typedef unsigned int BufferU32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ BufferU32x4 bufferLoadU32x4Coherent(BufferRsrc r, uint32_t off) {
   return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 16);
 }
+// Tells the optimiser that memory may have changed (other workgroups'
+// stores); no instruction.
+__device__ __forceinline__ void memoryChanged() { asm volatile("" ::: "memory"); }
 __device__ __forceinline__ void bufferStore(uint32_t v, BufferRsrc r, uint32_t off) {
   __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
 }

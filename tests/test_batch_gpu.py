@@ -68,3 +68,34 @@ def test_batch_single_large_mesh_matches_standalone(bvh, orc):
         b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=bits)
         b.build(use_graph=False)
         check_segments(bvh, orc, b, v, f, fc, vc, [0, 1, 2])
+
+
+def test_batch_wide_sort_tile_edges(bvh, orc):
+    """Meshes at the edges of the 8-bit pass's sort tiles (11,264 keys): one
+    tile exactly, one key more, 16 tiles (every predecessor in the first
+    look-back round) and one key more (a second round), with small meshes
+    between them; every segment against the oracle."""
+    tile = 1024 * 11
+    sizes = [tile, 1, tile + 1, 2, 16 * tile, 16 * tile + 1, 3 * tile - 1, 5]
+    parts = [bvh.scene_soup(n, 40 + i) for i, n in enumerate(sizes)]
+    v = np.concatenate([p[0] for p in parts]).astype(np.float32)
+    f = np.concatenate([p[1] for p in parts]).astype(np.uint32)
+    fc = np.array([len(p[1]) for p in parts], np.uint32)
+    vc = np.array([len(p[0]) for p in parts], np.uint32)
+    assert list(fc) == sizes
+    b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=8)
+    b.build(use_graph=False)
+    check_segments(bvh, orc, b, v, f, fc, vc, range(len(fc)))
+
+
+def test_batch_512_meshes_same_for_both_digit_widths(bvh):
+    """The 38.4M-triangle batch: the 8-bit passes (wide tiles) and the 10-bit
+    passes sort every mesh to the same order, so all 77M nodes agree."""
+    v, f, fc, vc = bvh.scene_batch(512, 1000, 1000, 500000)
+    out = []
+    for bits in (8, 10):
+        b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=bits)
+        b.build(use_graph=False)
+        out.append(b.nodes())
+        b.free()
+    assert out[0].tobytes() == out[1].tobytes()
