@@ -46,20 +46,28 @@ def kernel_bytes(name, n, nv, morton_bits, tile_log2, pass_idx=0, passes=0):
     key; 63-bit keys: the 5 passes over bits 24-62, all writing keys); group
     rank: key 8 + index 4 read, index 4 written (each group member read once
     per group, in cache; the index-based topology built here needs no sorted
-    keys, so none are written); level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
+    keys, so none are written).  63-bit keys of a full build below 2^24
+    triangles that fit one round of wide sort tiles are packed (the key's
+    low 24 bits replaced by the index): Morton writes the packed key 8 n and
+    the low bits 4 n, a onesweep pass reads and writes the 8-byte words only,
+    and the group rank reads the words 8 n and the low bits 4 n and writes
+    the index 4 n.  Level-0 build: sorted index 4 n + faces 12 n + vertices 12 nv read,
     48-byte leaves (n) and the inner nodes below the top tiles
     (n - ceil(n / 2^tile_log2)) written.  The MSD window and big-bucket sorts
     move a data-dependent share of the pairs, so they have no byte model."""
     kb = 8 if morton_bits == 63 else 4
+    packed = morton_bits == 63 and n < (1 << 24) and n <= 256 * 17408
     if name == "k_sceneBox":
         return 12 * n + 12 * nv
     if name == "k_vertBox":
         return 12 * nv
     if name in ("k_morton", "k_mortonMsd"):
-        return 12 * n + 12 * nv + kb * n
+        return 12 * n + 12 * nv + kb * n + (4 * n if packed and name == "k_morton" else 0)
     if name == "k_msdScatter":
         return kb * n + (kb + 4) * n
     if name.startswith("k_onesweep"):
+        if packed:
+            return 16 * n
         keep_last = morton_bits == 63  # 63-bit keys: the group step needs the sorted keys
         rd = kb + (0 if pass_idx == 0 else 4)
         wr = (0 if pass_idx == passes - 1 and not keep_last else kb) + 4

@@ -250,13 +250,27 @@ __global__ __launch_bounds__(kBoxBlock) void k_vertBox(const bvht_vec3 *__restri
 // block; block 0 records the box in prm) and the pass also writes its
 // blocks' partial triangle boxes (triParts) for k_mortonFix.  Otherwise the
 // mapping is prm's (the staged calls).
+// low != null (63-bit keys): each key is written packed, its low 24 bits
+// replaced by the triangle index, and the low bits go to low[index].
 constexpr int kMortonBlock = 1024;
+constexpr uint64_t kLowMask = (1ull << kSplitLoBits) - 1;
+template <typename K>
+__device__ __forceinline__ void storeKey(K *keys, uint32_t *low, uint32_t i, K code) {
+  if constexpr (sizeof(K) == 8) {
+    if (low) {
+      keys[i] = (code & ~kLowMask) | i;
+      low[i] = (uint32_t)(code & kLowMask);
+      return;
+    }
+  }
+  keys[i] = code;
+}
 static_assert(kMortonBlock == kBoxBlock, "the encode passes share the box reductions");
 template <typename K, int PASSES, int FIRST, bool VERTEX>
 __global__ __launch_bounds__(kMortonBlock) void k_morton(const bvht_vec3 *__restrict__ vert,
                                                          const bvht_Triangle *__restrict__ faces, uint32_t n,
                                                          BuildParams *__restrict__ prm, K *__restrict__ keys,
-                                                         uint32_t *__restrict__ hist,
+                                                         uint32_t *__restrict__ low, uint32_t *__restrict__ hist,
                                                          const float *__restrict__ vertParts, uint32_t numVertParts,
                                                          float *__restrict__ triParts) {
   __shared__ uint32_t s_hist[PASSES][kSortRadix];
@@ -303,7 +317,7 @@ __global__ __launch_bounds__(kMortonBlock) void k_morton(const bvht_vec3 *__rest
       code = morton63(c, off, sc);
     else
       code = morton30(c, off, sc);
-    if (valid) keys[i] = code;
+    if (valid) storeKey(keys, low, i, code);
 #pragma unroll
     for (int p = FIRST; p < PASSES; ++p)
       waveHistAdd(s_hist[p], (uint32_t)(code >> (p * kSortRadixBits)) & (kSortRadix - 1), valid);
@@ -324,12 +338,12 @@ __global__ __launch_bounds__(kMortonBlock) void k_morton(const bvht_vec3 *__rest
 // recomputed with the triangles' mapping and the histograms are corrected
 // by removing the old key's digits and adding the new one's.  Otherwise it
 // exits after the comparison.  prm is only read here (every block compares
-// against it).
+// against it).  The digits the sort uses are the same in packed keys.
 template <typename K, int PASSES, int FIRST>
 __global__ __launch_bounds__(kMortonBlock) void k_mortonFix(const bvht_vec3 *__restrict__ vert,
                                                             const bvht_Triangle *__restrict__ faces, uint32_t n,
                                                             BuildParams *__restrict__ prm, K *__restrict__ keys,
-                                                            uint32_t *__restrict__ hist,
+                                                            uint32_t *__restrict__ low, uint32_t *__restrict__ hist,
                                                             const float *__restrict__ triParts, uint32_t numTriParts) {
   __shared__ uint32_t s_hist[PASSES][kSortRadix];  // net change per bin (mod 2^32)
   float box[6];
@@ -352,7 +366,7 @@ __global__ __launch_bounds__(kMortonBlock) void k_mortonFix(const bvht_vec3 *__r
     else
       code = morton30(c, off, sc);
     const K old = keys[i];
-    keys[i] = code;
+    storeKey(keys, low, i, code);
 #pragma unroll
     for (int p = FIRST; p < PASSES; ++p) {
       atomicAdd(&s_hist[p][(uint32_t)(old >> (p * kSortRadixBits)) & (kSortRadix - 1)], 0xFFFFFFFFu);
@@ -484,6 +498,7 @@ struct Internal {
   SplitPlan split;  // digitBits 8 with 63-bit keys: top 30 bits, then the groups
   MsdPlan msd;      // MSD sort (digitBits 16)
   bool useMsd = false, useSplit = false;
+  bool packedKeys = false;  // keys[0] holds packed 63-bit keys (split_sort.hpp)
   float *partials = nullptr;
   uint32_t boxBlocks = 0;
   int resultBuf = 0;
@@ -586,28 +601,32 @@ bool launchMorton(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bool c
   const float *vparts = in->partials;
   const uint32_t nvparts = vertBlocks(mesh->numVert);
   float *tparts = in->partials + (size_t)kBoxBlocksMax * 6;
+  // the full build packs its 63-bit keys (the staged calls hand real keys
+  // to the caller)
+  in->packedKeys = check && in->useSplit && in->split.canPack;
+  uint32_t *low = in->packedKeys ? splitLow(in->split, (uint32_t *)mem->sortScratch) : nullptr;
   if (mem->opts.mortonBits == 63 && check)
     k_morton<uint64_t, 8, kSplitFirstPass, true><<<blocks, kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], hist, vparts, nvparts, tparts);
+        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], low, hist, vparts, nvparts, tparts);
   else if (mem->opts.mortonBits == 63)
     k_morton<uint64_t, 8, kSplitFirstPass, false><<<blocks, kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], hist, nullptr, 0u, nullptr);
+        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], nullptr, hist, nullptr, 0u, nullptr);
   else if (check)
     k_morton<uint32_t, 4, 0, true><<<blocks, kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], hist, vparts, nvparts, tparts);
+        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], nullptr, hist, vparts, nvparts, tparts);
   else
     k_morton<uint32_t, 4, 0, false><<<blocks, kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], hist, nullptr, 0u, nullptr);
+        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], nullptr, hist, nullptr, 0u, nullptr);
   if (!checkLaunch("k_morton")) return false;
   if (!check) return true;
   timing().record("k_morton");
   const uint32_t fixBlocks = std::min<uint32_t>(blocks, (uint32_t)numCUs());
   if (mem->opts.mortonBits == 63)
     k_mortonFix<uint64_t, 8, kSplitFirstPass><<<fixBlocks, kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], hist, tparts, blocks);
+        mesh->vert, mesh->faces, n, prm, (uint64_t *)mem->keys[0], low, hist, tparts, blocks);
   else
     k_mortonFix<uint32_t, 4, 0><<<fixBlocks, kMortonBlock, 0, stream()>>>(
-        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], hist, tparts, blocks);
+        mesh->vert, mesh->faces, n, prm, (uint32_t *)mem->keys[0], nullptr, hist, tparts, blocks);
   if (!checkLaunch("k_mortonFix")) return false;
   timing().record("k_mortonFix");
   return true;
@@ -629,7 +648,7 @@ bool runSort(bvht_LBVH_WorkingMemory *mem) {
   }
   if (in->useSplit) {
     uint64_t *k2[2] = {(uint64_t *)mem->keys[0], (uint64_t *)mem->keys[1]};
-    const int r = splitRun(in->split, k2, mem->values, (uint32_t *)mem->sortScratch, keepKeys,
+    const int r = splitRun(in->split, k2, mem->values, (uint32_t *)mem->sortScratch, keepKeys, in->packedKeys,
                            tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr,
                            tm.enabled ? [](const char *s) { timing().record(s); } : nullptr);
     if (r < 0) return false;
@@ -1046,7 +1065,7 @@ const char *bvht_LBVH_stageName(int i) {
 // cleared by the call itself (fill kernels, no memset nodes).
 
 constexpr uint32_t kNoAllocMaxFaces = 1u << 20;
-constexpr size_t kNoAllocBytes = 28u << 20;
+constexpr size_t kNoAllocBytes = 32u << 20;
 static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
 namespace bvht {

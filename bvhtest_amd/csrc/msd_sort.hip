@@ -662,7 +662,7 @@ bool bvht_sortKeyIndexPairs(const void *keys, uint32_t n, uint32_t keyBits, uint
     uint32_t *vv[2] = {v0, v1};
     ok = devMalloc((void **)&scratch, splitScratchBytes(p), "sort scratch") && splitReset(p, scratch) &&
          sortHistogram(p.plan, k0, scratch);
-    const int r = ok ? splitRun(p, kk, vv, scratch, true) : -1;
+    const int r = ok ? splitRun(p, kk, vv, scratch, true, false) : -1;
     uint32_t words[2] = {0, 0};
     ok = r >= 0 &&
          check(hipMemcpyAsync(&words[0], scratch + p.plan.counterOff + 8, 4, hipMemcpyDeviceToHost, stream()),

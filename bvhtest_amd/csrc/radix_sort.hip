@@ -217,13 +217,16 @@ __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict_
 //    0 and stores past it are dropped by the buffer's range check.
 //  - A 64-bit key is loaded whole and ranked on the 32-bit word that holds
 //    the pass's digit (a digit never straddles the words).
-//  - The identity values of a first pass are made when they are staged.
+//  - VALS: kValsLoaded (values move with the keys), kValsIdentity (the
+//    first pass: values are the input index, made when they are staged) or
+//    kValsNone (a keys-only sort).
 //  - The tile's global offsets are the sums of every earlier tile's
 //    published per-digit counts, loaded by all 1024 threads at once (16-byte
 //    loads of four digits, 8 in flight each) instead of a chained look-back.
 //    Tile ids come from `counter`, so every tile a workgroup waits for has
 //    started; the wait for a count is bounded and raises the error flag.
-template <typename K, bool IDENTITY>
+enum { kValsLoaded = 0, kValsIdentity = 1, kValsNone = 2 };
+template <typename K, int VALS>
 __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restrict__ keysIn,
                                                              const uint32_t *__restrict__ valsIn,
                                                              K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
@@ -321,11 +324,11 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
       }
     }
   }
-  uint32_t val[IDENTITY ? 1 : ITEMS];
-  if (!IDENTITY) {
+  uint32_t val[VALS == kValsLoaded ? ITEMS : 1];
+  if (VALS == kValsLoaded) {
     const BufferRsrc vals = bufferRsrc(valsIn, n * 4u);
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) val[IDENTITY ? 0 : i] = bufferLoadU32(vals, (first2 + i * 64) * 4u);
+    for (int i = 0; i < ITEMS; ++i) val[VALS == kValsLoaded ? i : 0] = bufferLoadU32(vals, (first2 + i * 64) * 4u);
   }
   __syncthreads();  // s_wave is free from here: it holds the prefix partials
 
@@ -394,13 +397,14 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
       if (keysOut && out[k] < n) bufferStore(kk, kout, out[k] * (uint32_t)sizeof(K));
     }
   }
+  if (VALS == kValsNone) return;
   __syncthreads();
   uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
   uint32_t first3 = first;
   recomputeAfter(first3);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
-    if (first3 + i * 64 < n) s_v[rank[i]] = IDENTITY ? first3 + i * 64 : val[IDENTITY ? 0 : i];
+    if (first3 + i * 64 < n) s_v[rank[i]] = VALS == kValsIdentity ? first3 + i * 64 : val[VALS == kValsLoaded ? i : 0];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k)
@@ -451,26 +455,34 @@ int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityVa
     setError("sort scratch must be 16-byte aligned");
     return -1;
   }
+  const bool keysOnly = vals == nullptr;
+  if (keysOnly && !(p.wide && p.keyBytes == 8 && keepKeys)) {
+    setError("keys-only sort: wide tiles and 64-bit keys only");
+    return -1;
+  }
   int cur = 0;
   for (int pass = firstPass; pass < p.passes; ++pass) {
     const uint32_t *hist = scratch + p.histOff + (size_t)pass * kSortRadix;
     uint32_t *counter = scratch + p.counterOff + pass;
     uint32_t *status = scratch + p.statusOff + (size_t)pass * p.numTiles * kSortRadix;
-    const uint32_t *vin = (pass == firstPass && identityValues) ? nullptr : vals[cur];
+    const uint32_t *vin = (keysOnly || (pass == firstPass && identityValues)) ? nullptr : vals[cur];
     bool last = pass == p.passes - 1;
     void *kout = (last && !keepKeys) ? nullptr : keys[cur ^ 1];
     uint32_t shift = pass * kSortRadixBits;
-    if (p.wide && p.keyBytes == 8 && !vin)
-      k_onesweepWide<uint64_t, true><<<p.numTiles, kWideBlock, 0, stream()>>>(
+    if (keysOnly)
+      k_onesweepWide<uint64_t, kValsNone><<<p.numTiles, kWideBlock, 0, stream()>>>(
+          (const uint64_t *)keys[cur], nullptr, (uint64_t *)kout, nullptr, p.n, shift, hist, status, counter);
+    else if (p.wide && p.keyBytes == 8 && !vin)
+      k_onesweepWide<uint64_t, kValsIdentity><<<p.numTiles, kWideBlock, 0, stream()>>>(
           (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else if (p.wide && p.keyBytes == 8)
-      k_onesweepWide<uint64_t, false><<<p.numTiles, kWideBlock, 0, stream()>>>(
+      k_onesweepWide<uint64_t, kValsLoaded><<<p.numTiles, kWideBlock, 0, stream()>>>(
           (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else if (p.wide && !vin)
-      k_onesweepWide<uint32_t, true><<<p.numTiles, kWideBlock, 0, stream()>>>(
+      k_onesweepWide<uint32_t, kValsIdentity><<<p.numTiles, kWideBlock, 0, stream()>>>(
           (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else if (p.wide)
-      k_onesweepWide<uint32_t, false><<<p.numTiles, kWideBlock, 0, stream()>>>(
+      k_onesweepWide<uint32_t, kValsLoaded><<<p.numTiles, kWideBlock, 0, stream()>>>(
           (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
     else if (p.keyBytes == 8)
       k_onesweep<uint64_t, kSortItems64><<<p.numTiles, kSortBlock, 0, stream()>>>(

@@ -48,8 +48,10 @@ bool sortReset(const SortPlan &p, uint32_t *scratch);
 // Morton kernel instead).
 bool sortHistogram(const SortPlan &p, const void *keys, uint32_t *scratch);
 // Runs every pass.  keys[0]/vals[0] hold the input (vals[0] may be null:
-// values are then the input index).  Returns the buffer index (0/1) holding
-// the result.  With keepKeys == false the last pass writes values only.
+// values are then the input index; vals itself null: a keys-only sort, for
+// wide plans of 64-bit keys with keepKeys).  Returns the buffer index (0/1)
+// holding the result.  With keepKeys == false the last pass writes values
+// only.
 // With firstPass > 0 the lower digits are left unsorted: the result is
 // ordered stably by key >> (8 firstPass).
 int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityValues, bool keepKeys,

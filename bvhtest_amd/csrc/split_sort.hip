@@ -49,15 +49,26 @@ __device__ __forceinline__ uint32_t groupEnd(const uint64_t *kf, uint32_t n, uin
   return lo + 1;
 }
 
+// Packed words (the fused build's keys, n < 2^24): a 63-bit key with its
+// low 24 bits replaced by the input index.  The top bits, which the
+// onesweep passes sort, are the key's; the low bits are kept in `low`,
+// indexed by input index.
+constexpr uint64_t kPackMask = (1ull << kSplitLoBits) - 1;
+__device__ __forceinline__ uint32_t packedIndex(uint64_t w) { return (uint32_t)(w & kPackMask); }
+__device__ __forceinline__ uint64_t packedKey(uint64_t w, const uint32_t *low) {
+  return (w & ~kPackMask) | low[packedIndex(w)];
+}
+
 // Rank of pair (key, v) inside its group [gs, ge) of the sorted-by-top-bits
 // array, read from global memory (the general path).
-__device__ __forceinline__ uint32_t rankGlobal(const uint64_t *kf, const uint32_t *vals, uint32_t gs, uint32_t ge,
-                                               uint64_t key, uint32_t v) {
+template <bool PACKED>
+__device__ __forceinline__ uint32_t rankGlobal(const uint64_t *kf, const uint32_t *vals, const uint32_t *low,
+                                               uint32_t gs, uint32_t ge, uint64_t key, uint32_t v) {
   uint32_t rank = 0;
 #pragma unroll 4
   for (uint32_t q = gs; q < ge; ++q) {
-    const uint64_t kq = kf[q];
-    const uint32_t vq = vals[q];
+    const uint64_t kq = PACKED ? packedKey(kf[q], low) : kf[q];
+    const uint32_t vq = PACKED ? packedIndex(kf[q]) : vals[q];
     rank += (kq < key || (kq == key && vq < v)) ? 1u : 0u;
   }
   return rank;
@@ -85,7 +96,12 @@ __device__ __forceinline__ int waveMaxInt(int x) {
 // general path: galloping search for the bounds, members read from global
 // memory (O(m) compares).  A group longer than kSplitGroupMax is listed once,
 // by its first pair, for k_splitLong.
+// PACKED: kf holds packed words and vals is unused; the full key of every
+// pair outside a one-pair group is gathered from `low` (a one-pair group
+// needs no compare).
+template <bool PACKED>
 __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ kf, const uint32_t *__restrict__ vals,
+                                                   const uint32_t *__restrict__ low,
                                                    uint32_t n, uint32_t *__restrict__ valsOut,
                                                    uint64_t *__restrict__ keysOut, uint32_t *__restrict__ ctrl,
                                                    uint2 *__restrict__ list, uint32_t listCap) {
@@ -103,7 +119,7 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
     const int64_t pos = (int64_t)P0 + 64 * (w - 1) + lane;
     const bool in = pos >= 0 && pos < (int64_t)n;
     k[w] = in ? kf[pos] : 0ull;
-    v[w] = in ? vals[pos] : 0u;
+    v[w] = PACKED ? packedIndex(k[w]) : (in ? vals[pos] : 0u);
     h[w] = in ? hiOf(k[w]) : (pos < 0 ? kBelow : kAbove);
   }
   // bit l of b[w]: a group starts at position P0 + 64 (w - 1) + l.  Whether
@@ -116,6 +132,17 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
     const uint64_t up = __shfl_up(h[w], 1, 64);
     const uint64_t before = w > 0 ? readLane64(h[w - 1], 63) : (P0 == 64 ? kBelow : h[0]);
     b[w] = __ballot(h[w] != (lane == 0 ? before : up));
+  }
+  if (PACKED) {
+    // full keys, except where a group starts here and at the next position
+    // (starts at window position 0 and past 191 are not known: gathered)
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      const int64_t pos = (int64_t)P0 + 64 * (w - 1) + lane;
+      const bool s0 = (b[w] >> lane) & 1ull;
+      const bool s1 = lane < 63 ? ((b[w] >> (lane + 1)) & 1ull) : (w < 2 ? (b[w + 1] & 1ull) : false);
+      if (pos >= 0 && pos < (int64_t)n && !(s0 && s1)) k[w] = packedKey(k[w], low);
+    }
   }
   const bool valid = P0 + lane < n;
   const uint64_t key = k[1];
@@ -173,10 +200,11 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
       return;
     }
     start = g0;
-    rank = rankGlobal(kf, vals, g0, g1, key, val);
+    rank = rankGlobal<PACKED>(kf, vals, low, g0, g1, key, val);
   }
   valsOut[start + rank] = val;
-  if (keysOut) keysOut[start + rank] = key;
+  // PACKED: `key` is already the full key unless the group has one pair
+  if (keysOut) keysOut[start + rank] = PACKED ? ((key & ~kPackMask) | low[val]) : key;
 }
 
 // Long groups: one workgroup per group, a stable LSD over the low 24 bits
@@ -186,8 +214,10 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
 // reverse, pass 3 again (kA, vA) -> (kB, vB).  Chunks of
 // kLongChunk pairs are ranked stably with wave ballots (item-major order in
 // a wave, waves in order) and placed after the digit's running offset.
-__global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *__restrict__ kA, uint32_t *__restrict__ vA,
-                                                          uint64_t *__restrict__ kB, uint32_t *__restrict__ vB,
+// PACKED: pass 1 reads packed words from kA (full keys from `low`).
+template <bool PACKED>
+__global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *kA, uint32_t *vA, uint64_t *kB, uint32_t *vB,
+                                                          const uint32_t *__restrict__ low,
                                                           const uint2 *__restrict__ list, uint32_t *__restrict__ ctrl,
                                                           uint32_t listCap) {
   constexpr int WAVES = kSortBlock / 64;
@@ -218,7 +248,9 @@ __global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *__restrict__
 #pragma unroll
         for (int i = 0; i < kLongItems; ++i) {
           const uint32_t j = c0 + tid + i * kSortBlock;
-          d[i] = j < m ? (uint32_t)(ki[s + j] >> shift) & (kSortRadix - 1) : 0xFFFFFFFFu;
+          const uint64_t kj = j < m ? ki[s + j] : 0ull;
+          d[i] = j < m ? (uint32_t)((PACKED && pass == 0 ? packedKey(kj, low) : kj) >> shift) & (kSortRadix - 1)
+                       : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int i = 0; i < kLongItems; ++i)
@@ -236,7 +268,12 @@ __global__ __launch_bounds__(kSortBlock) void k_splitLong(uint64_t *__restrict__
         for (int i = 0; i < kLongItems; ++i) {
           const uint32_t j = base + i * 64 + lane;
           key[i] = j < m ? ki[s + j] : 0ull;
-          val[i] = j < m ? vi[s + j] : 0u;
+          if (PACKED && pass == 0) {
+            val[i] = packedIndex(key[i]);
+            if (j < m) key[i] = packedKey(key[i], low);
+          } else {
+            val[i] = j < m ? vi[s + j] : 0u;
+          }
         }
         for (int w = 0; w < WAVES; ++w) s_wave[w][tid] = 0;
         __syncthreads();
@@ -286,10 +323,12 @@ SplitPlan makeSplitPlan(uint32_t n) {
   SplitPlan p;
   p.n = n;
   p.plan = makeSortPlan(n, 8, 63);
+  p.canPack = p.plan.wide && n < (1u << kSplitLoBits);
   p.listCap = n / (kSplitGroupMax + 1) + 1;
   p.ctrlOff = p.plan.totalWords;
   p.listOff = p.ctrlOff + 16;
-  p.totalWords = p.listOff + 2 * (size_t)p.listCap;
+  p.lowOff = p.listOff + 2 * (size_t)p.listCap;
+  p.totalWords = p.lowOff + (p.canPack ? n : 0);
   return p;
 }
 
@@ -297,22 +336,38 @@ bool splitReset(const SplitPlan &p, uint32_t *scratch) {
   return sortReset(p.plan, scratch) && fillU32(scratch + p.ctrlOff, 0u, 16);
 }
 
-int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch, bool keepKeys,
-             void (*afterPass)(int),
-             void (*afterStage)(const char *)) {
+int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t *scratch, bool keepKeys, bool packed,
+             void (*afterPass)(int), void (*afterStage)(const char *)) {
   if (p.n == 0) return 0;
+  if (packed && !p.canPack) {
+    setError("packed 63-bit keys need wide sort tiles and fewer than 2^24 keys");
+    return -1;
+  }
   void *kv[2] = {keys[0], keys[1]};
-  const int r = sortRun(p.plan, kv, vals, true, true, scratch, afterPass, kSplitFirstPass);
+  // packed words sort without values and carry the index in their low bits;
+  // either way the group step reads the sorted keys
+  const int r = sortRun(p.plan, kv, packed ? nullptr : vals, true, true, scratch, afterPass, kSplitFirstPass);
   if (r < 0) return -1;
   uint32_t *ctrl = scratch + p.ctrlOff;
   uint2 *list = reinterpret_cast<uint2 *>(scratch + p.listOff);
-  k_splitRank<<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], vals[r], p.n, vals[r ^ 1],
-                                                     keepKeys ? keys[r ^ 1] : nullptr, ctrl, list, p.listCap);
+  const uint32_t *low = packed ? splitLow(p, scratch) : nullptr;
+  uint64_t *keysOut = keepKeys ? keys[r ^ 1] : nullptr;
+  if (packed)
+    k_splitRank<true><<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], nullptr, low, p.n, vals[r ^ 1], keysOut, ctrl,
+                                                             list, p.listCap);
+  else
+    k_splitRank<false><<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], vals[r], nullptr, p.n, vals[r ^ 1], keysOut,
+                                                              ctrl, list, p.listCap);
   if (!checkLaunch("k_splitRank")) return -1;
   if (afterStage) afterStage("k_splitRank");
   // long groups are rare (none at the 4.24M-triangle scene): a small grid
   // keeps the launch cheap when the list is empty
-  k_splitLong<<<32, kSortBlock, 0, stream()>>>(keys[r], vals[r], keys[r ^ 1], vals[r ^ 1], list, ctrl, p.listCap);
+  if (packed)
+    k_splitLong<true><<<32, kSortBlock, 0, stream()>>>(keys[r], vals[r], keys[r ^ 1], vals[r ^ 1], low, list, ctrl,
+                                                       p.listCap);
+  else
+    k_splitLong<false><<<32, kSortBlock, 0, stream()>>>(keys[r], vals[r], keys[r ^ 1], vals[r ^ 1], nullptr, list,
+                                                        ctrl, p.listCap);
   if (!checkLaunch("k_splitLong")) return -1;
   if (afterStage) afterStage("k_splitLong");
   return r ^ 1;

@@ -257,7 +257,7 @@ uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem);
  * callers that cannot allocate device memory -- code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc: the same
  * build as bvht_LBVH_build with default options (mortonBits 30 or 63), its
- * working memory taken from a library-owned device array of 28 MiB per GPU
+ * working memory taken from a library-owned device array of 32 MiB per GPU
  * (held until the process ends).  Meshes of at most 2^20 faces; larger
  * ones are refused with an error.  The caller passes the device mesh and a
  * BVH allocated for it (LBVH_allocateBVH).  Calls that overlap on one GPU

@@ -143,6 +143,23 @@ def test_full_size_63bit_scene(bvh, orc):
         tree.free()
 
 
+def test_63bit_past_wide_tiles(bvh, orc):
+    """2,048 spheres + 800,000 soup triangles (4.54M): more keys than one round
+    of wide sort tiles holds (256 x 17,408 on MI355X), so the full build
+    sorts full keys with values (256-thread tiles) instead of packed keys."""
+    v, f = bvh.scene_spheres_soup(2048, 800000, 4)
+    print(f"scene n={len(f)}", flush=True)
+    assert len(f) > 256 * 17408
+    keys, sf, expect, _ = orc.lbvh(v, f, 63)
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, 63)
+    tree = b.allocate_bvh()
+    b.build(tree)
+    assert b.sort_status() == 0
+    assert np.array_equal(b.sorted_faces(), sf)
+    assert tree.download().tobytes() == expect.tobytes()
+
+
 class ShiftedMesh:
     """A MeshRaw whose vertex array starts 4 bytes into its allocation (not
     16-byte aligned)."""
