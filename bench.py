@@ -375,16 +375,17 @@ def batch_bench(bt, L, reps=5):
     for lb in builders:
         lb.free()
     # modelled bytes of the whole batched build (u32 keys, one pass per
-    # digit): mesh boxes 12 n + 12 V; Morton 12 n + 12 V + 4 n; each sort
-    # pass key + value read and written (the first reads no value, the last
-    # writes no key); level-0 build 16 n + 12 V + 48-byte leaves and inner
-    # nodes below the top tiles
+    # digit): mesh boxes 12 n + 12 V; Morton 12 n + 12 V + 4 n key + 24 n
+    # leaf-box record; each sort pass key + value read and written (the
+    # first reads no value, the last writes no key); level-0 build sorted
+    # index 4 n + record 24 n + 48-byte leaves and inner nodes below the top
+    # tiles
     nv_all = int(vc.sum())
     passes = 4 if int(b.mem.radixBits) == 8 else 3
     sort_b = sum(total * ((4 + (0 if p == 0 else 4)) + ((0 if p == passes - 1 else 4) + 4)) for p in range(passes))
     inner = sum(int(x) - -(-int(x) // (1 << int(best))) for x in fc)
-    batch_bytes = (12 * total + 12 * nv_all) + (16 * total + 12 * nv_all) + sort_b + (16 * total + 12 * nv_all
-                                                                                       + 48 * (total + inner))
+    batch_bytes = (12 * total + 12 * nv_all) + (40 * total + 12 * nv_all) + sort_b + (28 * total
+                                                                                      + 48 * (total + inner))
     batch_gbs = batch_bytes / (graph_ms * 1e-3) / 1e9
     out = {"meshes": int(len(fc)), "triangles": total, "tile_log2": int(best), "radix_bits": int(b.mem.radixBits),
            "roofline_end_to_end": {"bytes_per_tri": round(batch_bytes / total, 1), "gbs": round(batch_gbs, 1),

@@ -727,7 +727,8 @@ static bool treeLevelsT(bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_
     const uint64_t U = (((uint64_t)n) + (1ull << unitLog) - 1) >> unitLog;
     const uint32_t blocks = (uint32_t)((U + (1u << TLOG) - 1) >> TLOG);
     if (first)
-      k_buildLevel0<TLOG, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, sd, nullptr, nullptr);
+      k_buildLevel0<TLOG, false, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, nullptr, sd,
+                                                                              nullptr, nullptr);
     else
       k_buildLevelUp<TLOG, false><<<blocks, kBuildBlock, 0, stream()>>>(nodes, sd, nullptr, nullptr, unitLog);
     if (!checkLaunch(first ? "k_buildLevel0" : "k_buildLevelUp")) return false;
@@ -753,14 +754,17 @@ bool launchTreeLevels(int tlog, bvht_BVHNode *nodes, const uint32_t *sorted, con
 
 template <int TLOG>
 static bool treeLevelsBatchT(bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
-                             const bvht_vec3 *vert, const SegDesc *segs, const uint2 *const *tiles,
+                             const bvht_vec3 *vert, const float2 *rec, const SegDesc *segs, const uint2 *const *tiles,
                              const uint32_t *counts, int levels) {
   const SegDesc none = {0, 0, 0, 0};
   for (int L = 0; L < levels; ++L) {
     if (counts[L] == 0) continue;
-    if (L == 0)
-      k_buildLevel0<TLOG, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, none, segs,
-                                                                           tiles[L]);
+    if (L == 0 && rec)
+      k_buildLevel0<TLOG, true, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, rec, none,
+                                                                                 segs, tiles[L]);
+    else if (L == 0)
+      k_buildLevel0<TLOG, true, false><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, sorted, faces, vert, nullptr,
+                                                                                  none, segs, tiles[L]);
     else
       k_buildLevelUp<TLOG, true><<<counts[L], kBuildBlock, 0, stream()>>>(nodes, none, segs, tiles[L],
                                                                             (uint32_t)(L * TLOG));
@@ -770,13 +774,13 @@ static bool treeLevelsBatchT(bvht_BVHNode *nodes, const uint32_t *sorted, const 
 }
 
 bool launchTreeLevelsBatch(int tlog, bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
-                           const bvht_vec3 *vert, const SegDesc *segs, const uint2 *const *tiles,
+                           const bvht_vec3 *vert, const float2 *rec, const SegDesc *segs, const uint2 *const *tiles,
                            const uint32_t *counts, int levels) {
   switch (tlog) {
-    case 8: return treeLevelsBatchT<8>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
-    case 9: return treeLevelsBatchT<9>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
-    case 10: return treeLevelsBatchT<10>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
-    case 11: return treeLevelsBatchT<11>(nodes, sorted, faces, vert, segs, tiles, counts, levels);
+    case 8: return treeLevelsBatchT<8>(nodes, sorted, faces, vert, rec, segs, tiles, counts, levels);
+    case 9: return treeLevelsBatchT<9>(nodes, sorted, faces, vert, rec, segs, tiles, counts, levels);
+    case 10: return treeLevelsBatchT<10>(nodes, sorted, faces, vert, rec, segs, tiles, counts, levels);
+    case 11: return treeLevelsBatchT<11>(nodes, sorted, faces, vert, rec, segs, tiles, counts, levels);
   }
   setError("launchTreeLevelsBatch: unsupported tile log2 %d", tlog);
   return false;

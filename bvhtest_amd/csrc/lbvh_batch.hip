@@ -84,14 +84,23 @@ __host__ __device__ constexpr int segPasses(int radixBits) { return (30 + radixB
 
 // Morton keys of one chunk (up to 2^kChunkLog triangles of one mesh) and
 // the mesh's histograms of every pass (LDS, then one atomic per non-empty
-// bin).  Four triangles per thread per trip, loads issued before use.
+// bin).  Four triangles per thread per trip, loads issued before use.  The
+// triangle's leaf box (the tile build's, from the same vertices) goes to
+// rec, three float2 per triangle, so that the tile build gathers one
+// record per leaf instead of a face and three vertices.  A wave's 64
+// records are contiguous: they pass through 1.5 KB of LDS and are written
+// as 8-byte words by consecutive lanes (512 contiguous bytes per store
+// instruction instead of 8 bytes every 24).  Trips are wave-uniform, so the
+// exchange is inside the wave: its LDS operations complete in order, and
+// compiler barriers keep the stores and loads where they are.
 template <int RB>
 __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__restrict__ vertAll,
                                                            const bvht_Triangle *__restrict__ faces,
                                                            const SegDesc *__restrict__ segs,
                                                            const uint2 *__restrict__ chunks,
                                                            const uint32_t *__restrict__ boxOrd,
-                                                           uint32_t *__restrict__ keys, uint32_t *__restrict__ segHist) {
+                                                           uint32_t *__restrict__ keys, float2 *__restrict__ rec,
+                                                           uint32_t *__restrict__ segHist) {
   constexpr int R = 1 << RB, P = segPasses(RB);
   static_assert(P * R <= kSegMaxHist, "histogram fits the LDS array");
   __shared__ uint32_t s_hist[P * R];
@@ -111,6 +120,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
   const bvht_Triangle *fc = faces + sd.faceBase;
   const uint32_t p0 = c.y << kChunkLog, p1 = min(p0 + (1u << kChunkLog), sd.n);
   constexpr int U = 4;
+  __shared__ float2 s_rec[kBuildBlock / 64][64 * 3];
+  float2 *wrec = s_rec[threadIdx.x >> 6];
+  const uint32_t lane = threadIdx.x & 63;
   // trips are wave-uniform (the histogram adds use ballots)
   for (uint32_t p = p0 + threadIdx.x; p - (threadIdx.x & 63) < p1; p += U * kBuildBlock) {
     bvht_Triangle tri[U];
@@ -132,6 +144,22 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
       const bool valid = q < p1;
       const uint32_t key = morton30(centroid(va[u], vb[u], vc[u]), off, sc);
       if (valid) keys[sd.faceBase + q] = key;
+      {
+        const bvht_AABB b = triangleBox(va[u], vb[u], vc[u]);
+        wrec[3 * lane] = make_float2(b.minMax[0].x, b.minMax[0].y);
+        wrec[3 * lane + 1] = make_float2(b.minMax[0].z, b.minMax[1].x);
+        wrec[3 * lane + 2] = make_float2(b.minMax[1].y, b.minMax[1].z);
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        // the wave's valid triangles are its first m lanes (q grows with the lane)
+        const uint32_t m = (uint32_t)__popcll(__ballot(valid));
+        float2 *dst = rec + 3 * (size_t)(sd.faceBase + q - lane);
+#pragma unroll
+        for (uint32_t j = lane; j < 3 * 64; j += 64)
+          if (j < 3 * m) dst[j] = wrec[j];
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+      }
 #pragma unroll
       for (int pass = 0; pass < P; ++pass) waveHistAdd(s_hist + pass * R, (key >> (pass * RB)) & (R - 1), valid);
     }
@@ -494,6 +522,7 @@ struct BatchInternal {
   uint32_t *boxOrd = nullptr;
   uint32_t *keys[2] = {nullptr, nullptr};
   uint32_t *vals[2] = {nullptr, nullptr};
+  float2 *rec = nullptr;  // leaf box per triangle (three float2), written by k_segMorton
   int radixBits = 8;
   uint2 *chunks = nullptr;     // (mesh, chunk) of the Morton pass
   uint32_t numChunks = 0;
@@ -556,14 +585,14 @@ bool batchStagesT(BatchInternal *in, const bvht_vec3 *vert, const bvht_Triangle 
   int r = 0;
   if (in->radixBits == 10) {
     k_segMorton<10><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
-                                                                 in->keys[0], in->sortScratch + 16);
+                                                                 in->keys[0], in->rec, in->sortScratch + 16);
     if (!checkLaunch("k_segMorton") || !segSort<10>(in, r)) return false;
   } else {
     k_segMorton<8><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
-                                                                in->keys[0], in->sortScratch + 16);
+                                                                in->keys[0], in->rec, in->sortScratch + 16);
     if (!checkLaunch("k_segMorton") || !segSort<8>(in, r)) return false;
   }
-  return launchTreeLevelsBatch(TLOG, nodes, in->vals[r], faces, vert, in->segs, in->tiles.data(),
+  return launchTreeLevelsBatch(TLOG, nodes, in->vals[r], faces, vert, in->rec, in->segs, in->tiles.data(),
                                in->tileCounts.data(), (int)in->tiles.size());
 }
 
@@ -710,6 +739,7 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
       !devAlloc(&in->boxOrd, 6ull * numMeshes, "batch boxes") || !devAlloc(&in->keys[0], in->totalFaces, "batch keys") ||
       !devAlloc(&in->keys[1], in->totalFaces, "batch keys") || !devAlloc(&in->vals[0], in->totalFaces, "batch values") ||
       !devAlloc(&in->vals[1], in->totalFaces, "batch values") ||
+      !devAlloc(&in->rec, 3ull * in->totalFaces, "batch leaf boxes") ||
       !devAlloc(&in->sortScratch, 16 + in->histWords + in->statusWords, "batch sort") ||
       !check(hipStreamCreateWithFlags(&in->capStream, hipStreamNonBlocking), "batch capture stream")) {
     bvht_LBVH_freeBatch(&m);
@@ -794,6 +824,7 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem) {
     devFree(in->keys[1]);
     devFree(in->vals[0]);
     devFree(in->vals[1]);
+    devFree(in->rec);
     devFree(in->sortScratch);
     delete in;
   }

@@ -161,11 +161,15 @@ __device__ __forceinline__ void storeInner(bvht_BVHNode *nodes, uint32_t l, uint
 // Six workgroups per CU (at most 80 VGPRs) measure faster than four (the
 // compiler's 105 VGPRs without the bound) and than eight (64 VGPRs, with
 // spills).
-template <int TLOG, bool BATCH>
+// REC: the leaf boxes are read from `rec` (three float2 per triangle, by
+// global face index, written by the key pass) instead of being made from a
+// face and three vertices: one gather per leaf instead of four.
+template <int TLOG, bool BATCH, bool REC>
 __global__ __launch_bounds__(kBuildBlock, 6) void k_buildLevel0(bvht_BVHNode *__restrict__ nodesAll,
                                                              const uint32_t *__restrict__ sorted,
                                                              const bvht_Triangle *__restrict__ faces,
-                                                             const bvht_vec3 *__restrict__ vertAll, SegDesc single,
+                                                             const bvht_vec3 *__restrict__ vertAll,
+                                                             const float2 *__restrict__ rec, SegDesc single,
                                                              const SegDesc *__restrict__ segs,
                                                              const uint2 *__restrict__ tiles) {
   constexpr uint32_t T = 1u << TLOG;
@@ -201,15 +205,28 @@ __global__ __launch_bounds__(kBuildBlock, 6) void k_buildLevel0(bvht_BVHNode *__
     f[i] = k < cnt ? sorted[sd.faceBase + u0 + k] - sd.faceBase : 0u;
     if (f[i] >= n) f[i] = 0;  // never read outside the mesh, even from a corrupt permutation
   }
-  bvht_Triangle t[R];
+  bvht_AABB lbs[R];  // every leaf box first: the loads die before the barriers below
+  if constexpr (REC) {
 #pragma unroll
-  for (int i = 0; i < R; ++i) t[i] = faces[sd.faceBase + f[i]];
-  bvht_vec3 va[R], vb[R], vc[R];
+    for (int i = 0; i < R; ++i) {
+      const float2 *r = rec + 3 * (size_t)(sd.faceBase + f[i]);
+      const float2 a = r[0], b = r[1], c = r[2];
+      lbs[i].minMax[0] = {a.x, a.y, b.x};
+      lbs[i].minMax[1] = {b.y, c.x, c.y};
+    }
+  } else {
+    bvht_Triangle t[R];
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    va[i] = vert[t[i].v1];
-    vb[i] = vert[t[i].v2];
-    vc[i] = vert[t[i].v3];
+    for (int i = 0; i < R; ++i) t[i] = faces[sd.faceBase + f[i]];
+    bvht_vec3 va[R], vb[R], vc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      va[i] = vert[t[i].v1];
+      vb[i] = vert[t[i].v2];
+      vc[i] = vert[t[i].v3];
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) lbs[i] = triangleBox(va[i], vb[i], vc[i]);
   }
   // leaves first: the gathered vertices are dead before the reductions start.
   // A wave's 64 leaves (positions k - lane .. k - lane + 63) are written
@@ -220,9 +237,6 @@ __global__ __launch_bounds__(kBuildBlock, 6) void k_buildLevel0(bvht_BVHNode *__
   uint8_t *stageOk = s_stageOk[threadIdx.x >> 6];
   const uint32_t lane = threadIdx.x & 63;
   float leaf[R][6];
-  bvht_AABB lbs[R];  // every leaf box first: the vertices die before the barriers below
-#pragma unroll
-  for (int i = 0; i < R; ++i) lbs[i] = triangleBox(va[i], vb[i], vc[i]);
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const uint32_t k = threadIdx.x + i * kBuildBlock, p = u0 + k;
@@ -387,9 +401,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_buildLevelUp(bvht_BVHNode *__re
 bool launchTreeLevels(int tlog, bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
                       const bvht_vec3 *vert, uint32_t n, void (*afterLevel)(bool level0));
 
-// Batched variant: tilesPerLevel[L] lists (segment, group) pairs of level L.
+// Batched variant: tilesPerLevel[L] lists (segment, group) pairs of level L;
+// rec (if not null) holds every triangle's leaf box (k_buildLevel0's REC).
 bool launchTreeLevelsBatch(int tlog, bvht_BVHNode *nodes, const uint32_t *sorted, const bvht_Triangle *faces,
-                           const bvht_vec3 *vert, const SegDesc *segs, const uint2 *const *tilesPerLevel,
-                           const uint32_t *tilesCount, int levels);
+                           const bvht_vec3 *vert, const float2 *rec, const SegDesc *segs,
+                           const uint2 *const *tilesPerLevel, const uint32_t *tilesCount, int levels);
 
 }  // namespace bvht
