@@ -375,7 +375,7 @@ def batch_bench(bt, L, reps=5):
     for lb in builders:
         lb.free()
     # modelled bytes of the whole batched build (u32 keys, one pass per
-    # digit): mesh boxes 12 n + 12 V; Morton 12 n + 12 V + 4 n key + 24 n
+    # digit): mesh boxes 12 V (every vertex once); Morton 12 n + 12 V + 4 n key + 24 n
     # leaf-box record; each sort pass key + value read and written (the
     # first reads no value, the last writes no key); level-0 build sorted
     # index 4 n + record 24 n + 48-byte leaves and inner nodes below the top
@@ -384,7 +384,7 @@ def batch_bench(bt, L, reps=5):
     passes = 4 if int(b.mem.radixBits) == 8 else 3
     sort_b = sum(total * ((4 + (0 if p == 0 else 4)) + ((0 if p == passes - 1 else 4) + 4)) for p in range(passes))
     inner = sum(int(x) - -(-int(x) // (1 << int(best))) for x in fc)
-    batch_bytes = (12 * total + 12 * nv_all) + (40 * total + 12 * nv_all) + sort_b + (28 * total
+    batch_bytes = 12 * nv_all + (40 * total + 12 * nv_all) + sort_b + (28 * total
                                                                                       + 48 * (total + inner))
     batch_gbs = batch_bytes / (graph_ms * 1e-3) / 1e9
     out = {"meshes": int(len(fc)), "triangles": total, "tile_log2": int(best), "radix_bits": int(b.mem.radixBits),

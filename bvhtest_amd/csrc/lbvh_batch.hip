@@ -4,11 +4,17 @@
 // Every segment's output equals the standalone build of that mesh (own scene
 // box, local face and node indices): the reference build of
 // lib/builder/LBVH_CUDA.cu applied per mesh.  Stages:
-//   k_segBox     per tile of one mesh: triangle boxes, block reduce, 6 atomic
-//                min/max (order-preserving u32 encoding) into the mesh box;
+//   k_segInit    zeroes the sort scratch, initialises the mesh boxes;
+//   k_segVertBox per run of one mesh's vertices: block reduce, 6 atomic
+//                min/max (order-preserving u32 encoding) into the mesh's
+//                vertex box (the reference's triangle box once shifted by
+//                -+FLT_EPSILON, unless an unreferenced vertex lies outside);
 //   k_segMorton  per chunk of one mesh: the mesh's offset/scale (as
-//                LBVH_CUDA.cu:171-180), the 30-bit Morton code, and the
-//                mesh's digit histograms of every sort pass;
+//                LBVH_CUDA.cu:171-180), the 30-bit Morton code, the
+//                triangle's leaf box, the mesh's digit histograms of every
+//                sort pass and the box of the referenced vertices;
+//   k_segFix     per chunk: the two mesh boxes compared; a mesh whose
+//                boxes differ gets its keys and histograms recomputed;
 //   k_segPrefix  per (mesh, pass): exclusive scan of the histogram;
 //   k_segOnesweep one segmented onesweep pass: sort tiles never cross a
 //                mesh, the look-back stops at the mesh's first tile, and
@@ -29,28 +35,31 @@ namespace bvht {
 
 namespace {
 
-constexpr uint32_t kOrdZero = 0x80000000u;  // f2o(+0.0f)
+// Per-mesh boxes, kBoxWords order-preserving words per mesh: [0, 6) the min
+// and max of all the mesh's vertices (k_segVertBox), [6, 12) those of the
+// vertices its triangles reference (k_segMorton).  The reference's scene
+// box (triangle boxes, each its vertices' min - FLT_EPSILON and max +
+// FLT_EPSILON, with the origin included: LBVH_CUDA.cu:60-119) is the
+// second pair shifted (f32 rounding is monotone); the keys are made from
+// the first, which equals it unless an unreferenced vertex lies outside
+// the triangles' box, and k_segFix repairs such meshes.
+constexpr int kBoxWords = 12;
+constexpr uint32_t kVertChunk = 4096;  // vertices per k_segVertBox workgroup
 
-template <int TLOG>
-__global__ __launch_bounds__(kBuildBlock) void k_segBox(const bvht_vec3 *__restrict__ vertAll,
-                                                        const bvht_Triangle *__restrict__ faces,
-                                                        const SegDesc *__restrict__ segs, const uint2 *__restrict__ tiles,
-                                                        uint32_t *__restrict__ boxOrd) {
-  const uint2 t = tiles[blockIdx.x];
-  const SegDesc sd = segs[t.x];
-  const bvht_vec3 *vert = vertAll + sd.vertBase;
-  const uint32_t p0 = t.y << TLOG, p1 = min(p0 + (1u << TLOG), sd.n);
-  float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};  // the zero box of LBVH_CUDA.cu:104-106
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBuildBlock) {
-    const bvht_Triangle tri = faces[sd.faceBase + p];
-    const bvht_AABB b = triangleBox(vert[tri.v1], vert[tri.v2], vert[tri.v3]);
-    mn[0] = mergeMin(mn[0], b.minMax[0].x);
-    mn[1] = mergeMin(mn[1], b.minMax[0].y);
-    mn[2] = mergeMin(mn[2], b.minMax[0].z);
-    mx[0] = mergeMax(mx[0], b.minMax[1].x);
-    mx[1] = mergeMax(mx[1], b.minMax[1].y);
-    mx[2] = mergeMax(mx[2], b.minMax[1].z);
+__device__ __forceinline__ uint32_t boxInitWord(uint32_t w) { return (w % 6) < 3 ? f2o(HUGE_VALF) : f2o(-HUGE_VALF); }
+
+// The reference's box from a raw vertex min / max pair (6 words at w):
+// -+FLT_EPSILON, then the origin.
+__device__ __forceinline__ void shiftedMeshBox(const uint32_t *w, float m[6]) {
+  for (int a = 0; a < 3; ++a) {
+    m[a] = mergeMin(0.0f, o2f(w[a]) - kFltEpsilon);
+    m[3 + a] = mergeMax(0.0f, o2f(w[3 + a]) + kFltEpsilon);
   }
+}
+
+// Block reduction of per-thread min / max; threads 0..5 add the result to
+// the 6 order-preserving words at dst with atomics.
+__device__ void blockBoxAtomics(const float mn[3], const float mx[3], uint32_t *dst) {
   __shared__ float s_red[6][kBuildBlock / 64];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int a = 0; a < 3; ++a) {
@@ -64,12 +73,44 @@ __global__ __launch_bounds__(kBuildBlock) void k_segBox(const bvht_vec3 *__restr
   if (threadIdx.x < 6) {
     const int a = threadIdx.x;
     float v = s_red[a][0];
-    for (int w = 1; w < kBuildBlock / 64; ++w) v = a < 3 ? mergeMin(v, s_red[a][w]) : mergeMax(v, s_red[a][w]);
+    for (int w = 1; w < kBuildBlock / 64; ++w) v = a < 3 ? fminf(v, s_red[a][w]) : fmaxf(v, s_red[a][w]);
     if (a < 3)
-      atomicMin(boxOrd + 6 * t.x + a, f2o(v));
+      atomicMin(dst + a, f2o(v));
     else
-      atomicMax(boxOrd + 6 * t.x + a, f2o(v));
+      atomicMax(dst + a, f2o(v));
   }
+}
+
+// Zeroes the sort scratch and initialises the per-mesh boxes (one launch).
+__global__ void k_segInit(uint32_t *__restrict__ scratch, size_t scratchWords, uint32_t *__restrict__ boxOrd,
+                          size_t boxWords) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < scratchWords + boxWords;
+       i += (size_t)gridDim.x * blockDim.x) {
+    if (i < scratchWords)
+      scratch[i] = 0;
+    else
+      boxOrd[i - scratchWords] = boxInitWord((uint32_t)((i - scratchWords) % kBoxWords));
+  }
+}
+
+// Min / max of a run of up to kVertChunk vertices of one mesh (vchunks:
+// mesh, first vertex, count) into the mesh's first box.
+__global__ __launch_bounds__(kBuildBlock) void k_segVertBox(const bvht_vec3 *__restrict__ vertAll,
+                                                            const uint4 *__restrict__ vchunks,
+                                                            uint32_t *__restrict__ boxOrd) {
+  const uint4 c = vchunks[blockIdx.x];
+  float mn[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, mx[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+#pragma unroll 4
+  for (uint32_t i = threadIdx.x; i < c.z; i += kBuildBlock) {
+    const bvht_vec3 p = vertAll[c.y + i];
+    mn[0] = fminf(mn[0], p.x);
+    mn[1] = fminf(mn[1], p.y);
+    mn[2] = fminf(mn[2], p.z);
+    mx[0] = fmaxf(mx[0], p.x);
+    mx[1] = fmaxf(mx[1], p.y);
+    mx[2] = fmaxf(mx[2], p.z);
+  }
+  blockBoxAtomics(mn, mx, boxOrd + (size_t)kBoxWords * c.x);
 }
 
 constexpr int kSegItems = 15;                               // keys per thread of a sort tile
@@ -97,8 +138,7 @@ template <int RB>
 __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__restrict__ vertAll,
                                                            const bvht_Triangle *__restrict__ faces,
                                                            const SegDesc *__restrict__ segs,
-                                                           const uint2 *__restrict__ chunks,
-                                                           const uint32_t *__restrict__ boxOrd,
+                                                           const uint2 *__restrict__ chunks, uint32_t *boxOrd,
                                                            uint32_t *__restrict__ keys, float2 *__restrict__ rec,
                                                            uint32_t *__restrict__ segHist) {
   constexpr int R = 1 << RB, P = segPasses(RB);
@@ -110,12 +150,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
   bvht_vec3 off, sc;
   {
     float m[6];
-    for (int a = 0; a < 6; ++a) m[a] = o2f(boxOrd[6 * c.x + a]);
+    shiftedMeshBox(boxOrd + (size_t)kBoxWords * c.x, m);
     // LBVH_calcMortonCodes host part (LBVH_CUDA.cu:171-180), f32, IEEE division
     off = {0.0f - m[0], 0.0f - m[1], 0.0f - m[2]};
     sc = {__fdiv_rn(1.0f, m[3] + off.x), __fdiv_rn(1.0f, m[4] + off.y), __fdiv_rn(1.0f, m[5] + off.z)};
   }
   __syncthreads();
+  // min / max of the referenced vertices (a lane past the chunk repeats triangle p0)
+  float rmn[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, rmx[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
   const bvht_vec3 *vert = vertAll + sd.vertBase;
   const bvht_Triangle *fc = faces + sd.faceBase;
   const uint32_t p0 = c.y << kChunkLog, p1 = min(p0 + (1u << kChunkLog), sd.n);
@@ -142,6 +184,12 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
     for (int u = 0; u < U; ++u) {
       const uint32_t q = p + u * kBuildBlock;
       const bool valid = q < p1;
+      rmn[0] = fminf(rmn[0], fminf(fminf(va[u].x, vb[u].x), vc[u].x));
+      rmn[1] = fminf(rmn[1], fminf(fminf(va[u].y, vb[u].y), vc[u].y));
+      rmn[2] = fminf(rmn[2], fminf(fminf(va[u].z, vb[u].z), vc[u].z));
+      rmx[0] = fmaxf(rmx[0], fmaxf(fmaxf(va[u].x, vb[u].x), vc[u].x));
+      rmx[1] = fmaxf(rmx[1], fmaxf(fmaxf(va[u].y, vb[u].y), vc[u].y));
+      rmx[2] = fmaxf(rmx[2], fmaxf(fmaxf(va[u].z, vb[u].z), vc[u].z));
       const uint32_t key = morton30(centroid(va[u], vb[u], vc[u]), off, sc);
       if (valid) keys[sd.faceBase + q] = key;
       {
@@ -169,6 +217,55 @@ __global__ __launch_bounds__(kBuildBlock) void k_segMorton(const bvht_vec3 *__re
   for (int i = threadIdx.x; i < P * R; i += kBuildBlock) {
     const uint32_t n = s_hist[i];
     if (n) atomicAdd(h + i, n);
+  }
+  blockBoxAtomics(rmn, rmx, boxOrd + (size_t)kBoxWords * c.x + 6);
+}
+
+// Compares a chunk's mesh's two boxes (shifted: the boxes the keys were
+// made from and the reference's); when they differ (an unreferenced vertex
+// outside the triangles' box), recomputes the chunk's keys with the
+// triangles' box and corrects the mesh's histograms (old digits out, new
+// in).  Otherwise it exits after the comparison.  The leaf-box records do
+// not depend on the box and stay.
+template <int RB>
+__global__ __launch_bounds__(kBuildBlock) void k_segFix(const bvht_vec3 *__restrict__ vertAll,
+                                                        const bvht_Triangle *__restrict__ faces,
+                                                        const SegDesc *__restrict__ segs,
+                                                        const uint2 *__restrict__ chunks,
+                                                        const uint32_t *__restrict__ boxOrd,
+                                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ segHist) {
+  constexpr int R = 1 << RB, P = segPasses(RB);
+  const uint2 c = chunks[blockIdx.x];
+  float vb[6], tb[6];
+  shiftedMeshBox(boxOrd + (size_t)kBoxWords * c.x, vb);
+  shiftedMeshBox(boxOrd + (size_t)kBoxWords * c.x + 6, tb);
+  bool same = true;
+  for (int a = 0; a < 6; ++a) same = same && __float_as_uint(vb[a]) == __float_as_uint(tb[a]);
+  if (same) return;  // block-uniform
+  __shared__ uint32_t s_hist[P * R];  // net change per bin (mod 2^32)
+  for (int i = threadIdx.x; i < P * R; i += kBuildBlock) s_hist[i] = 0;
+  __syncthreads();
+  const SegDesc sd = segs[c.x];
+  const bvht_vec3 off = {0.0f - tb[0], 0.0f - tb[1], 0.0f - tb[2]};
+  const bvht_vec3 sc = {__fdiv_rn(1.0f, tb[3] + off.x), __fdiv_rn(1.0f, tb[4] + off.y), __fdiv_rn(1.0f, tb[5] + off.z)};
+  const bvht_vec3 *vert = vertAll + sd.vertBase;
+  const uint32_t p0 = c.y << kChunkLog, p1 = min(p0 + (1u << kChunkLog), sd.n);
+  for (uint32_t q = p0 + threadIdx.x; q < p1; q += kBuildBlock) {
+    const bvht_Triangle t = faces[sd.faceBase + q];
+    const uint32_t key = morton30(centroid(vert[t.v1], vert[t.v2], vert[t.v3]), off, sc);
+    const uint32_t old = keys[sd.faceBase + q];
+    keys[sd.faceBase + q] = key;
+#pragma unroll
+    for (int pass = 0; pass < P; ++pass) {
+      atomicAdd(&s_hist[pass * R + ((old >> (pass * RB)) & (R - 1))], 0xFFFFFFFFu);
+      atomicAdd(&s_hist[pass * R + ((key >> (pass * RB)) & (R - 1))], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t *h = segHist + (size_t)c.x * (P * R);
+  for (int i = threadIdx.x; i < P * R; i += kBuildBlock) {
+    const uint32_t d = s_hist[i];
+    if (d) atomicAdd(h + i, d);
   }
 }
 
@@ -526,6 +623,8 @@ struct BatchInternal {
   int radixBits = 8;
   uint2 *chunks = nullptr;     // (mesh, chunk) of the Morton pass
   uint32_t numChunks = 0;
+  uint4 *vchunks = nullptr;    // (mesh, first vertex, count) of the vertex box pass
+  uint32_t numVChunks = 0;
   uint2 *sortTiles = nullptr;  // (mesh, tile) of the sort passes
   uint32_t numSortTiles = 0;
   uint2 *wideTiles = nullptr;  // (mesh, tile) of the wide 8-bit passes
@@ -576,21 +675,32 @@ bool segSort(BatchInternal *in, int &result) {
 
 template <int TLOG>
 bool batchStagesT(BatchInternal *in, const bvht_vec3 *vert, const bvht_Triangle *faces, bvht_BVHNode *nodes) {
-  if (!fillU32(in->boxOrd, kOrdZero, 6ull * in->M)) return false;
   const int P = segPasses(in->radixBits), R = 1 << in->radixBits;
   const uint32_t sortTiles = useWideSegSort(in) ? in->numWideTiles : in->numSortTiles;
-  if (!fillU32(in->sortScratch, 0u, 16 + (size_t)in->M * P * R + (size_t)P * sortTiles * R)) return false;
-  k_segBox<TLOG><<<in->tileCounts[0], kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->tiles[0], in->boxOrd);
-  if (!checkLaunch("k_segBox")) return false;
+  const size_t scratchWords = 16 + (size_t)in->M * P * R + (size_t)P * sortTiles * R;
+  const size_t boxWords = (size_t)kBoxWords * in->M;
+  const uint32_t initBlocks = (uint32_t)std::min<size_t>(divUp(scratchWords + boxWords, 256), (size_t)numCUs() * 8);
+  k_segInit<<<initBlocks, 256, 0, stream()>>>(in->sortScratch, scratchWords, in->boxOrd, boxWords);
+  if (!checkLaunch("k_segInit")) return false;
+  if (in->numVChunks) {
+    k_segVertBox<<<in->numVChunks, kBuildBlock, 0, stream()>>>(vert, in->vchunks, in->boxOrd);
+    if (!checkLaunch("k_segVertBox")) return false;
+  }
   int r = 0;
   if (in->radixBits == 10) {
     k_segMorton<10><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
                                                                  in->keys[0], in->rec, in->sortScratch + 16);
-    if (!checkLaunch("k_segMorton") || !segSort<10>(in, r)) return false;
+    if (!checkLaunch("k_segMorton")) return false;
+    k_segFix<10><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
+                                                              in->keys[0], in->sortScratch + 16);
+    if (!checkLaunch("k_segFix") || !segSort<10>(in, r)) return false;
   } else {
     k_segMorton<8><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
                                                                 in->keys[0], in->rec, in->sortScratch + 16);
-    if (!checkLaunch("k_segMorton") || !segSort<8>(in, r)) return false;
+    if (!checkLaunch("k_segMorton")) return false;
+    k_segFix<8><<<in->numChunks, kBuildBlock, 0, stream()>>>(vert, faces, in->segs, in->chunks, in->boxOrd,
+                                                             in->keys[0], in->sortScratch + 16);
+    if (!checkLaunch("k_segFix") || !segSort<8>(in, r)) return false;
   }
   return launchTreeLevelsBatch(TLOG, nodes, in->vals[r], faces, vert, in->rec, in->segs, in->tiles.data(),
                                in->tileCounts.data(), (int)in->tiles.size());
@@ -712,6 +822,12 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
   }
   // Morton chunks and sort tiles: per mesh, never crossing one
   std::vector<uint2> chunks, sortTiles, wideTiles;
+  std::vector<uint4> vchunks;
+  for (uint32_t i = 0; i < numMeshes; ++i) {
+    const uint32_t nv = vertCounts[i], vb = in->segsHost[i].vertBase;
+    for (uint32_t c = 0; (uint64_t)c * kVertChunk < nv; ++c)
+      vchunks.push_back(make_uint4(i, vb + c * kVertChunk, std::min(kVertChunk, nv - c * kVertChunk), 0u));
+  }
   for (uint32_t i = 0; i < numMeshes; ++i) {
     const uint32_t n = faceCounts[i];
     for (uint32_t c = 0; (uint64_t)c << kChunkLog < n; ++c) chunks.push_back(make_uint2(i, c));
@@ -719,6 +835,7 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
     for (uint32_t t = 0; (uint64_t)t * kSegWideTile < n; ++t) wideTiles.push_back(make_uint2(i, t));
   }
   in->numChunks = (uint32_t)chunks.size();
+  in->numVChunks = (uint32_t)vchunks.size();
   in->numSortTiles = (uint32_t)sortTiles.size();
   in->numWideTiles = (uint32_t)wideTiles.size();
   // scratch for the larger of the two digit widths (10 bits: 3 passes x 1024)
@@ -731,12 +848,15 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
       !check(hipMemcpy(in->sortTiles, sortTiles.data(), sizeof(uint2) * sortTiles.size(), hipMemcpyHostToDevice),
              "batch sort tiles") ||
       !devAlloc(&in->wideTiles, wideTiles.size(), "batch wide sort tiles") ||
+      !devAlloc(&in->vchunks, std::max<size_t>(vchunks.size(), 1), "batch vertex chunks") ||
+      !check(hipMemcpy(in->vchunks, vchunks.data(), sizeof(uint4) * vchunks.size(), hipMemcpyHostToDevice),
+             "batch vertex chunks") ||
       !check(hipMemcpy(in->wideTiles, wideTiles.data(), sizeof(uint2) * wideTiles.size(), hipMemcpyHostToDevice),
              "batch wide sort tiles") ||
       !devAlloc(&in->segs, numMeshes, "batch segments") ||
       !check(hipMemcpy(in->segs, in->segsHost.data(), sizeof(SegDesc) * numMeshes, hipMemcpyHostToDevice),
              "batch segments") ||
-      !devAlloc(&in->boxOrd, 6ull * numMeshes, "batch boxes") || !devAlloc(&in->keys[0], in->totalFaces, "batch keys") ||
+      !devAlloc(&in->boxOrd, (size_t)kBoxWords * numMeshes, "batch boxes") || !devAlloc(&in->keys[0], in->totalFaces, "batch keys") ||
       !devAlloc(&in->keys[1], in->totalFaces, "batch keys") || !devAlloc(&in->vals[0], in->totalFaces, "batch values") ||
       !devAlloc(&in->vals[1], in->totalFaces, "batch values") ||
       !devAlloc(&in->rec, 3ull * in->totalFaces, "batch leaf boxes") ||
@@ -818,6 +938,7 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem) {
     devFree(in->chunks);
     devFree(in->sortTiles);
     devFree(in->wideTiles);
+    devFree(in->vchunks);
     devFree(in->segs);
     devFree(in->boxOrd);
     devFree(in->keys[0]);

@@ -99,3 +99,36 @@ def test_batch_512_meshes_same_for_both_digit_widths(bvh):
         out.append(b.nodes())
         b.free()
     assert out[0].tobytes() == out[1].tobytes()
+
+
+@pytest.mark.parametrize("bits", [8, 10])
+def test_batch_vertex_box_check_and_repair(bvh, orc, bits):
+    """Each mesh's box comes from all its vertices and is checked against
+    the box of the vertices its triangles reference (the reference's scene
+    box).  Meshes with an unreferenced vertex outside (past the max, below
+    the min, many of them) take the repair; one inside, or none, does not.
+    Every segment against the oracle, twice in a row (a repair does not
+    leak into the next build), with and without the graph."""
+    rng = np.random.default_rng(11)
+    parts = []
+    for i in range(8):
+        v, f = bvh.scene_heightfield(48 + 8 * i, 24, 20 + i, 0.2)
+        lo, hi = v.min(axis=0), v.max(axis=0)
+        if i == 1:
+            v = np.concatenate([v, [[5.0, 3.0, 7.0]]])
+        elif i == 3:
+            v, f = np.concatenate([[[-2.0, -1.0, -3.0]], v]), f + 1
+        elif i == 5:
+            v = np.concatenate([v, [(lo + hi) / 2]])
+        elif i == 6:
+            v = np.concatenate([v, rng.random((1001, 3), dtype=np.float32) * 9 - 4])
+        parts.append((v.astype(np.float32), f.astype(np.uint32)))
+    v = np.concatenate([p[0] for p in parts]).astype(np.float32)
+    f = np.concatenate([p[1] for p in parts]).astype(np.uint32)
+    fc = np.array([len(p[1]) for p in parts], np.uint32)
+    vc = np.array([len(p[0]) for p in parts], np.uint32)
+    b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=bits)
+    for graph in (False, False, True, True):
+        b.nb.upload(np.zeros(int(b.mem.totalNodes), bvh.NODE_DTYPE))
+        b.build(use_graph=graph)
+        check_segments(bvh, orc, b, v, f, fc, vc, range(len(fc)))
