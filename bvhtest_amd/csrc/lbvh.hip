@@ -32,7 +32,9 @@ namespace bvht {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTileLog = 10;  // default: 1024 units per build workgroup
+// default: 256 sorted positions per build workgroup (at 4.24M triangles the
+// build takes 0.350 ms with 2^8, 0.361 with 2^9, 0.363 with 2^10, 0.490 with 2^11)
+constexpr int kTileLog = 8;
 // Scene box: 1024-thread workgroups (16 waves hide the face -> vertex load
 // chain), two per CU.
 constexpr int kBoxBlock = 1024;

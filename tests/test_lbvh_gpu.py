@@ -61,11 +61,12 @@ def test_fused_build_bit_exact(bvh, orc, cases, name, bits, digits):
     assert nodes_equal(got, expect)
 
 
-@pytest.mark.parametrize("tile", [8, 9, 11])
+@pytest.mark.parametrize("tile", [9, 10, 11])
 @pytest.mark.parametrize("name", ["soup2", "soup17", "soup1000", "soup1025", "soup4097", "soup100003", "flat_dupes"])
 def test_tile_sizes_bit_exact(bvh, orc, cases, name, tile):
-    # tiles of 2^8..2^11 sorted positions, each cut at n: the inner nodes of
-    # a tile come from the lanes of their indices and from the upper levels
+    # tiles of 2^9..2^11 sorted positions (2^8 is the default every other test
+    # builds with), each cut at n: the inner nodes of a tile come from the
+    # lanes of their indices and from the upper levels
     v, f = cases[name]
     print(f"case {name} n={len(f)} tile=2^{tile}", flush=True)
     keys, sf, expect, scene = orc.lbvh(v, f, 30)
