@@ -134,14 +134,20 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
     b[w] = __ballot(h[w] != (lane == 0 ? before : up));
   }
   if (PACKED) {
-    // full keys, except where a group starts here and at the next position
-    // (starts at window position 0 and past 191 are not known: gathered)
-#pragma unroll
-    for (int w = 0; w < 3; ++w) {
-      const int64_t pos = (int64_t)P0 + 64 * (w - 1) + lane;
-      const bool s0 = (b[w] >> lane) & 1ull;
-      const bool s1 = lane < 63 ? ((b[w] >> (lane + 1)) & 1ull) : (w < 2 ? (b[w + 1] & 1ull) : false);
-      if (pos >= 0 && pos < (int64_t)n && !(s0 && s1)) k[w] = packedKey(k[w], low);
+    // Full keys where a compare can read them: the wave's own positions
+    // outside one-pair groups, and the members of the groups that reach
+    // from the wave's first position back into the window before and from
+    // its last position into the window after.
+    const bool s0 = (b[1] >> lane) & 1ull;
+    const bool s1 = lane < 63 ? ((b[1] >> (lane + 1)) & 1ull) : (b[2] & 1ull) != 0;
+    if (P0 + lane < n && !(s0 && s1)) k[1] = packedKey(k[1], low);
+    if (!(b[1] & 1ull) && b[0]) {  // the first group starts in the window before, at its last start
+      const uint32_t from = 63 - __clzll((long long)b[0]);
+      if (lane >= from) k[0] = packedKey(k[0], low);  // positions P0 - 64 + lane >= 0 here
+    }
+    if (!(b[2] & 1ull)) {  // the last group reaches into the window after, up to its first start
+      const uint32_t upto = b[2] ? (uint32_t)__ffsll((long long)b[2]) - 1 : 64u;
+      if (lane < upto && (uint64_t)P0 + 64 + lane < n) k[2] = packedKey(k[2], low);
     }
   }
   const bool valid = P0 + lane < n;
