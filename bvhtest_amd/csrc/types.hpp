@@ -147,11 +147,26 @@ BVHT_HD uint32_t morton30(const bvht_vec3 &c, const bvht_vec3 &off, const bvht_v
   return x * 4 + y * 2 + z;
 }
 
+// expandBits21 as two 32-bit words: the low 11 bits spread to bits 0..30
+// (expandBits10's steps place bit 10 at bit 30 as well), the high 10 bits
+// to bits 33..60, i.e. bits 1..28 of the high word.  Equal to expandBits21
+// for every 21-bit input (checked exhaustively); 32-bit operations only.
+BVHT_HD void expandBits21Halves(uint32_t v, uint32_t &lo, uint32_t &hi) {
+  lo = expandBits10(v & 0x7FFu);
+  hi = expandBits10(v >> 11) << 1;
+}
+
+// x * 4 + y * 2 + z of the three spread coordinates (their bits are
+// disjoint, so the sum is an OR), assembled from 32-bit words.  The
+// quantised value is below 2^21, so its conversion goes through u32.
 BVHT_HD uint64_t morton63(const bvht_vec3 &c, const bvht_vec3 &off, const bvht_vec3 &scale) {
-  uint64_t x = expandBits21((uint64_t)quantize(c.x, off.x, scale.x, 2097152.0f, 2097151.0f));
-  uint64_t y = expandBits21((uint64_t)quantize(c.y, off.y, scale.y, 2097152.0f, 2097151.0f));
-  uint64_t z = expandBits21((uint64_t)quantize(c.z, off.z, scale.z, 2097152.0f, 2097151.0f));
-  return x * 4 + y * 2 + z;
+  uint32_t xl, xh, yl, yh, zl, zh;
+  expandBits21Halves((uint32_t)quantize(c.x, off.x, scale.x, 2097152.0f, 2097151.0f), xl, xh);
+  expandBits21Halves((uint32_t)quantize(c.y, off.y, scale.y, 2097152.0f, 2097151.0f), yl, yh);
+  expandBits21Halves((uint32_t)quantize(c.z, off.z, scale.z, 2097152.0f, 2097151.0f), zl, zh);
+  const uint32_t lo = (xl << 2) | (yl << 1) | zl;
+  const uint32_t hi = (xh << 2) | (xl >> 30) | (yh << 1) | (yl >> 31) | zh;
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // ---------------------------------------------------------------------------
