@@ -186,7 +186,11 @@ __global__ __launch_bounds__(kBuildBlock, 6) void k_buildLevel0(bvht_BVHNode *__
     group = t.y;
   } else {
     sd = single;
-    group = blockIdx.x;
+    // consecutive tiles to one XCD: workgroups are handed to the 8 XCDs
+    // round-robin, and neighbouring tiles share face and vertex lines (a
+    // bijection for any grid)
+    const uint32_t G = gridDim.x, q = G / 8, r = G % 8, x = blockIdx.x % 8;
+    group = x * q + min(x, r) + blockIdx.x / 8;
   }
   const uint32_t n = sd.n, N1 = n - 1;
   bvht_BVHNode *__restrict__ nodes = nodesAll + sd.nodeBase;
