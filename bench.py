@@ -235,6 +235,17 @@ def run_bench(args):
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dom, bits), "bytes_per_launch": int(dom_bytes),
                 "ms_per_launch": round(dom_ms, 4)}
+    # every modelled kernel the same way, most time per build first
+    roofline_kernels = {}
+    for fam in sorted(family, key=lambda k: -sum(ms for ms, _ in family[k])):
+        ls = family[fam]
+        f_ms = sum(ms for ms, _ in ls) / len(ls)
+        f_bytes = sum(b for _, b in ls) / len(ls)
+        f_gbs = f_bytes / (f_ms * 1e-3) / 1e9
+        roofline_kernels[fam] = {"launches": len(ls), "ms_per_build": round(sum(ms for ms, _ in ls), 4),
+                                 "ms_per_launch": round(f_ms, 4), "bytes_per_launch": int(f_bytes),
+                                 "gbs": round(f_gbs, 1), "frac": round(f_gbs / HBM_PEAK_GBS, 4),
+                                 "traffic": measured_traffic(fam, bits)}
     print(f"# stages (ms): {json.dumps({k: round(x, 4) for k, x in stages.items()})}", file=sys.stderr, flush=True)
 
     # per-stage and end-to-end fractions of the HBM peak: modelled bytes over
@@ -329,7 +340,7 @@ def run_bench(args):
                       "sort": ("MSD: 16-bit buckets, in-LDS sorts by (key, index)" if builder.mem.opts.digitBits == 16
                                else "onesweep over bits 24-62 (8-bit digits), then (key, index) rank in groups of "
                                     "equal top bits" if bits == 63 else "onesweep, 8-bit digits")},
-           "roofline": roofline, "roofline_stages": stage_roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
+           "roofline": roofline, "roofline_kernels": roofline_kernels, "roofline_stages": stage_roofline, "stages_ms": {k: round(x, 4) for k, x in stages.items()},
            "trace": trace, "bittner": bittner, "batched": batch, "learned_priority": learned, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
