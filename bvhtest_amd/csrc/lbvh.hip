@@ -1026,7 +1026,7 @@ uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem) {
   }
   const Internal *in = internalOf(mem);
   const uint32_t *sc = (const uint32_t *)mem->sortScratch;
-  const size_t word = in->useMsd ? in->msd.ctrlOff + 2 : (in->useSplit ? in->split.plan.counterOff : in->plan.counterOff) + 8;
+  const size_t word = in->useMsd ? in->msd.ctrlOff + 2 : (in->useSplit ? in->split.plan.errOff : in->plan.errOff);
   uint32_t v = 0xFFFFFFFFu, w = 0;
   if (!check(hipMemcpyAsync(&v, sc + word, 4, hipMemcpyDeviceToHost, stream()), "bvht_LBVH_sortStatus") ||
       (in->useSplit &&

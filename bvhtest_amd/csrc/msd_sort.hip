@@ -665,7 +665,7 @@ bool bvht_sortKeyIndexPairs(const void *keys, uint32_t n, uint32_t keyBits, uint
     const int r = ok ? splitRun(p, kk, vv, scratch, true, false) : -1;
     uint32_t words[2] = {0, 0};
     ok = r >= 0 &&
-         check(hipMemcpyAsync(&words[0], scratch + p.plan.counterOff + 8, 4, hipMemcpyDeviceToHost, stream()),
+         check(hipMemcpyAsync(&words[0], scratch + p.plan.errOff, 4, hipMemcpyDeviceToHost, stream()),
                "sort status") &&
          check(hipMemcpyAsync(&words[1], scratch + p.ctrlOff + 2, 4, hipMemcpyDeviceToHost, stream()), "sort status") &&
          check(hipStreamSynchronize(stream()), "sort sync");

@@ -42,7 +42,7 @@ template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
                                                          K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                          uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
-                                                         uint32_t *status, uint32_t *counter) {
+                                                         uint32_t *status, uint32_t *counter, uint32_t *err) {
   constexpr int WAVES = kSortBlock / 64;
   constexpr int TILE = kSortBlock * ITEMS;
   static_assert(kSortBlock == kSortRadix, "one digit per thread");
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict_
         // the tile counter starts at zero) ends the pass with an error flag
         // instead of a hang
         if (++spins > (1u << 24)) {
-          atomicOr(counter + 8, 1u);
+          atomicOr(err, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
                                                              uint32_t n, uint32_t shift,
                                                              const uint32_t *__restrict__ hist,
                                                              uint32_t *status,
-                                                             uint32_t *counter) {
+                                                             uint32_t *counter, uint32_t *err) {
   constexpr int WAVES = kWideBlock / 64;
   constexpr int ITEMS = kWideItems;
   constexpr uint32_t TILE = kWideTile;
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
         while (((v[0] & ~kCountMask) == 0 || (v[1] & ~kCountMask) == 0 || (v[2] & ~kCountMask) == 0 ||
                 (v[3] & ~kCountMask) == 0)) {
           if (++spins > (1u << 24)) {
-            atomicOr(counter + 8, 1u);
+            atomicOr(err, 1u);
             v = BufferU32x4{kFlagAgg, kFlagAgg, kFlagAgg, kFlagAgg};
             break;
           }
@@ -426,11 +426,12 @@ SortPlan makeSortPlan(uint32_t n, int keyBytes, int keyBits) {
   p.tile = p.wide ? kWideTile : kSortBlock * (keyBytes == 8 ? kSortItems64 : kSortItems32);
   p.numTiles = n == 0 ? 0 : divUp(n, p.tile);
   p.histWords = (size_t)p.passes * kSortRadix;
-  p.counterWords = 16;
+  p.counterWords = 16;  // a tile counter per pass (at most 8), then the error word
   p.statusWords = (size_t)p.passes * p.numTiles * kSortRadix;
   p.histOff = 0;
   p.counterOff = p.histOff + p.histWords;
   p.statusOff = p.counterOff + p.counterWords;
+  p.errOff = p.counterOff + 8;
   p.totalWords = p.statusOff + p.statusWords;
   return p;
 }
@@ -464,6 +465,7 @@ int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityVa
   for (int pass = firstPass; pass < p.passes; ++pass) {
     const uint32_t *hist = scratch + p.histOff + (size_t)pass * kSortRadix;
     uint32_t *counter = scratch + p.counterOff + pass;
+    uint32_t *err = scratch + p.errOff;  // one word for every pass
     uint32_t *status = scratch + p.statusOff + (size_t)pass * p.numTiles * kSortRadix;
     const uint32_t *vin = (keysOnly || (pass == firstPass && identityValues)) ? nullptr : vals[cur];
     bool last = pass == p.passes - 1;
@@ -471,25 +473,25 @@ int sortRun(const SortPlan &p, void *keys[2], uint32_t *vals[2], bool identityVa
     uint32_t shift = pass * kSortRadixBits;
     if (keysOnly)
       k_onesweepWide<uint64_t, kValsNone><<<p.numTiles, kWideBlock, 0, stream()>>>(
-          (const uint64_t *)keys[cur], nullptr, (uint64_t *)kout, nullptr, p.n, shift, hist, status, counter);
+          (const uint64_t *)keys[cur], nullptr, (uint64_t *)kout, nullptr, p.n, shift, hist, status, counter, err);
     else if (p.wide && p.keyBytes == 8 && !vin)
       k_onesweepWide<uint64_t, kValsIdentity><<<p.numTiles, kWideBlock, 0, stream()>>>(
-          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter, err);
     else if (p.wide && p.keyBytes == 8)
       k_onesweepWide<uint64_t, kValsLoaded><<<p.numTiles, kWideBlock, 0, stream()>>>(
-          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter, err);
     else if (p.wide && !vin)
       k_onesweepWide<uint32_t, kValsIdentity><<<p.numTiles, kWideBlock, 0, stream()>>>(
-          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter, err);
     else if (p.wide)
       k_onesweepWide<uint32_t, kValsLoaded><<<p.numTiles, kWideBlock, 0, stream()>>>(
-          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter, err);
     else if (p.keyBytes == 8)
       k_onesweep<uint64_t, kSortItems64><<<p.numTiles, kSortBlock, 0, stream()>>>(
-          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+          (const uint64_t *)keys[cur], vin, (uint64_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter, err);
     else
       k_onesweep<uint32_t, kSortItems32><<<p.numTiles, kSortBlock, 0, stream()>>>(
-          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter);
+          (const uint32_t *)keys[cur], vin, (uint32_t *)kout, vals[cur ^ 1], p.n, shift, hist, status, counter, err);
     if (!checkLaunch("onesweep pass")) return -1;
     if (afterPass) afterPass(pass);
     cur ^= 1;

@@ -36,6 +36,7 @@ struct SortPlan {
   // u32 word offsets into the scratch buffer
   size_t histWords = 0, counterWords = 0, statusWords = 0;
   size_t histOff = 0, counterOff = 0, statusOff = 0, totalWords = 0;
+  size_t errOff = 0;  // the error word: nonzero after a bounded wait gave up (any pass)
 };
 
 SortPlan makeSortPlan(uint32_t n, int keyBytes, int keyBits);
