@@ -473,10 +473,11 @@ class Bittner13GPU:
         return sahs
 
     def stats(self):
-        out = (ctypes.c_uint32 * 7)()
-        lib().bvht_bn13_stats(ctypes.byref(self.mem), out, 7)
+        out = (ctypes.c_uint32 * 8)()
+        lib().bvht_bn13_stats(ctypes.byref(self.mem), out, 8)
         return dict(conflicts=out[0], failed=out[1], contained=out[2], overflows=out[3], applied=out[4],
-                    unresolved=out[5], runaway=out[6], skipped=lib().calcNumSkipped(ctypes.byref(self.mem)))
+                    unresolved=out[5], runaway=out[6], sort_errors=out[7],
+                    skipped=lib().calcNumSkipped(ctypes.byref(self.mem)))
 
     def free(self):
         if self.mem is not None and self.mem.result:
@@ -618,6 +619,10 @@ class BatchBuilder:
         ms = (ctypes.c_float * 8)()
         best = lib().bvht_LBVH_autotuneBatch(ctypes.byref(self.mem), self.vb.ptr, self.fb.ptr, self.nb.ptr, ms)
         return best, list(ms)
+
+    def status(self):
+        """Error bits of the last batched sort: 0 = complete, 1 = a look-back wait gave up."""
+        return lib().bvht_LBVH_batchStatus(ctypes.byref(self.mem))
 
     def offsets(self, i):
         fo, vo, no = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()

@@ -121,6 +121,7 @@ SIGNATURES = {
     "bvht_LBVH_buildBatch": (c_bool, [P(LBVHBatchWorkingMemory), c_vp, c_vp, c_vp, c_bool]),
     "bvht_LBVH_freeBatch": (None, [P(LBVHBatchWorkingMemory)]),
     "bvht_LBVH_batchOffsets": (c_bool, [P(LBVHBatchWorkingMemory), c_u32, P(c_u32), P(c_u32), P(c_u32)]),
+    "bvht_LBVH_batchStatus": (c_u32, [P(LBVHBatchWorkingMemory)]),
     "bvht_LBVH_autotuneBatch": (ctypes.c_int, [P(LBVHBatchWorkingMemory), c_vp, c_vp, c_vp, P(c_f)]),
     "allocateMemory": (GPUWorkingMemory, [P(CUDAMemoryBVHPointer), c_u32, c_u32]),
     "freeMemory": (None, [P(GPUWorkingMemory)]),

@@ -44,7 +44,7 @@ constexpr int kStackCap = 1024;  // per-search frontier (findNode1 heap: 4096)
 constexpr uint32_t kAltSlots = 16;  // findNode2's array (CUDA_ALT_QUEUE_SIZE, Bittner13CUDA.hpp:28)
 
 // Stats words (device)
-enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_UNRESOLVED, ST_RUNAWAY, ST_WORDS = 8 };
+enum { ST_CONFLICT = 0, ST_FAILED, ST_CONTAIN, ST_OVERFLOW, ST_APPLIED, ST_UNRESOLVED, ST_RUNAWAY, ST_SORTERR, ST_WORDS = 8 };
 
 // Patch states and node locks.  A lock holds key(round, rank): the upper word
 // is ~round, so later rounds and applied patches (round kRoundWin, upper word
@@ -452,13 +452,15 @@ __global__ __launch_bounds__(64) void k_search(const bvht_BVHNode *__restrict__ 
                                                const uint32_t *__restrict__ order, const uint32_t *__restrict__ inner,
                                                uint32_t numInner, uint32_t chunk, uint32_t numChunks, uint32_t K,
                                                uint32_t offsetAccess, uint32_t alt, PatchRec *__restrict__ recs,
-                                               uint64_t *locks, uint32_t *__restrict__ stats, RecordSink sink) {
+                                               uint64_t *locks, uint32_t *__restrict__ stats, RecordSink sink,
+                                               const uint32_t *__restrict__ sortErr) {
   __shared__ Patch P;
   __shared__ uint32_t s_node[kStackCap];
   __shared__ float s_cost[kStackCap];
   __shared__ uint32_t s_top;
   __shared__ uint32_t s_ok, s_ins[2], s_unused[2], s_gp, s_best[2];
   const uint32_t k = blockIdx.x, lane = threadIdx.x;
+  if (sortErr && k == 0 && lane == 0 && *sortErr) atomicAdd(stats + ST_SORTERR, 1u);
   PatchRec *rec = recs + k;
   const uint32_t pos = PROBE ? order[k] : order[offsetAccess ? k * numChunks + chunk : chunk * K + k];
   if (pos >= numInner) {  // block-uniform; only a bad probe list gets here
@@ -1466,9 +1468,12 @@ void bn13_rmAndReinsChunk(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer
              K, in->K, in->numInner);
     return;
   }
+  // the first chunk of a step also counts a failed priority sort (its error
+  // word is cleared with the sort state at the next selection)
   k_search<false><<<K, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, in->sorted, in->inner, in->numInner, chunk,
                                           cfg.numChunks, K, cfg.offsetAccess ? 1u : 0u, cfg.altFindNode ? 1u : 0u,
-                                          in->recs, in->locks, in->stats, sinkOf(in));
+                                          in->recs, in->locks, in->stats, sinkOf(in),
+                                          chunk == 0 ? in->sortScratch + in->plan.errOff : nullptr);
   if (!checkLaunch("k_search")) return;
   k_commitFirst<<<divUp(K, 256), 256, 0, stream()>>>(in->recs, K, in->locks, in->marks, in->skipped, in->stats,
                                                      in->pending[0], in->numPending);
@@ -1541,7 +1546,7 @@ int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords) {
   if (!check(hipMemcpyAsync(h, in->stats, sizeof(h), hipMemcpyDeviceToHost, stream()), "bn13 stats") ||
       !check(hipStreamSynchronize(stream()), "bn13 stats sync"))
     return 0;
-  int n = maxWords < 7 ? maxWords : 7;
+  int n = maxWords < 8 ? maxWords : 8;
   for (int i = 0; i < n; ++i) out[i] = h[i];
   return n;
 }
@@ -1623,7 +1628,7 @@ bool bvht_bn13_probe(bvht_GPUWorkingMemory *data, bvht_CUDAMemoryBVHPointer *bvh
   if (!features(in, bvh)) return false;
   k_search<true><<<num, 64, 0, stream()>>>(bvh->nodes, in->root, in->N, devPositions, in->inner, in->numInner, 0, 1,
                                           num, 0u, cfg.altFindNode ? 1u : 0u, in->recs, in->locks, in->stats,
-                                          sinkOf(in));
+                                          sinkOf(in), nullptr);
   return checkLaunch("k_search (probe)");
 }
 

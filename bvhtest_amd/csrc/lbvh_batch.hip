@@ -952,6 +952,20 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem) {
   std::memset(mem, 0, sizeof(*mem));
 }
 
+// Error word of the last batched sort: bit 0 = a look-back wait gave up.
+uint32_t bvht_LBVH_batchStatus(const bvht_LBVH_BatchWorkingMemory *mem) {
+  if (!mem || !mem->internal) {
+    setError("bvht_LBVH_batchStatus: working memory not allocated");
+    return 0xFFFFFFFFu;
+  }
+  const BatchInternal *in = reinterpret_cast<const BatchInternal *>(mem->internal);
+  uint32_t v = 0xFFFFFFFFu;
+  if (!check(hipMemcpyAsync(&v, in->sortScratch + 15, 4, hipMemcpyDeviceToHost, stream()), "bvht_LBVH_batchStatus") ||
+      !check(hipStreamSynchronize(stream()), "bvht_LBVH_batchStatus sync"))
+    return 0xFFFFFFFFu;
+  return v;
+}
+
 // Offsets of mesh i inside the concatenated arrays.
 bool bvht_LBVH_batchOffsets(const bvht_LBVH_BatchWorkingMemory *mem, uint32_t i, uint32_t *faceOffset,
                             uint32_t *vertOffset, uint32_t *nodeOffset) {

@@ -314,6 +314,11 @@ bool bvht_LBVH_buildBatch(bvht_LBVH_BatchWorkingMemory *mem, const bvht_vec3 *ve
 void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem);
 bool bvht_LBVH_batchOffsets(const bvht_LBVH_BatchWorkingMemory *mem, uint32_t i, uint32_t *faceOffset,
                             uint32_t *vertOffset, uint32_t *nodeOffset);
+/* Error bits of the last batched sort (extension, no counterpart in the
+ * reference; synchronises): 0 = the sort ran to completion, 1 = a look-back
+ * wait gave up (the trees of that build are not valid).  0xFFFFFFFF on a
+ * bad argument or a failed copy. */
+uint32_t bvht_LBVH_batchStatus(const bvht_LBVH_BatchWorkingMemory *mem);
 /* Autotuner of the batched build: times every tile size 2^8..2^11 with both
  * digit widths (8 and 10 bits) on the caller's input and keeps the fastest
  * pair in mem->tileLog2 / mem->radixBits (every choice gives the same trees).
@@ -422,8 +427,10 @@ uint32_t calcNumSkipped(bvht_GPUWorkingMemory *data);
  * subtree another patch moves, [3] search frontier overflows, [4] applied
  * patches, [5] patches the conflict resolution left undecided (0 unless the
  * device breaks memory ordering; they are dropped), [6] searches stopped at
- * the step bound (0 for a tree; the patch is dropped).  Returns the number of
- * words written (at most 7). */
+ * the step bound (0 for a tree; the patch is dropped), [7] steps whose
+ * priority sort gave up a bounded wait (0 unless the device breaks memory
+ * ordering; that step's order is then wrong).  Returns the number of words
+ * written (at most 8). */
 int bvht_bn13_stats(bvht_GPUWorkingMemory *data, uint32_t *out, int maxWords);
 /* Weighted SAH of the device tree (lib/base/BVH.cpp:23-37 semantics:
  * sum over non-root nodes of SA * (leaf ? cLeaf : cInner) / SA(root)),

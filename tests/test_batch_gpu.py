@@ -8,6 +8,7 @@ pytestmark = pytest.mark.gpu
 
 
 def check_segments(bvh, orc, b, v, f, fc, vc, which):
+    assert b.status() == 0, "a look-back wait of the batched sort gave up"
     nodes = b.nodes()
     for i in which:
         fo, vo, no = b.offsets(i)
@@ -96,6 +97,7 @@ def test_batch_512_meshes_same_for_both_digit_widths(bvh):
     for bits in (8, 10):
         b = bvh.BatchBuilder(v, f, fc, vc, radix_bits=bits)
         b.build(use_graph=False)
+        assert b.status() == 0
         out.append(b.nodes())
         b.free()
     assert out[0].tobytes() == out[1].tobytes()

@@ -105,7 +105,7 @@ def replay_check(bvh, orc, v, f, steps, chunks, batch, search, **opts):
                 raise AssertionError(f"device tree differs from the replay after step {s} chunk {c}")
     st = opt.stats()
     print("stats", st, "replay", rep_stats, flush=True)
-    assert st["runaway"] == 0 and st["unresolved"] == 0
+    assert st["runaway"] == 0 and st["unresolved"] == 0 and st["sort_errors"] == 0
     assert (st["conflicts"], st["failed"], st["contained"]) == tuple(int(x) for x in rep_stats[:3])
     check_tree(orc, tree.download(), v, f)
     return st
