@@ -10,6 +10,9 @@ namespace {
 enum : uint32_t { kErrList = 1 };
 constexpr int kLongItems = 16;  // pairs per thread per chunk of the long-group sort
 constexpr uint32_t kLongChunk = kSortBlock * kLongItems;
+// 64-position chunks per wave in k_splitRank (4.24M 63-bit keys: 1 chunk
+// 31.6 us, 2 chunks 29.0, 4 chunks 35.1, 8 chunks 54.2)
+constexpr int kSplitChunks = 2;
 
 __device__ __forceinline__ uint64_t hiOf(uint64_t k) { return k >> kSplitLoBits; }
 
@@ -86,36 +89,39 @@ __device__ __forceinline__ int waveMaxInt(int x) {
 }
 
 // Each pair of a group of equal top bits (m <= kSplitGroupMax) goes to
-// group start + its rank by (key, index) inside the group.  A wave owns 64
-// consecutive positions and also loads the 64 before and the 64 after
-// (coalesced; neighbouring waves read them too, from cache).  Group
-// boundaries in those 192 positions are three ballots, so a group that
-// starts and ends inside them is found with bit scans and ranked against
-// its members in an LDS copy of the 192 positions, with no dependent global
-// loads.  A pair whose group reaches past them takes the
+// group start + its rank by (key, index) inside the group.  A wave owns C
+// chunks of 64 consecutive positions and also loads the 64 positions before
+// its first chunk and the 64 after its last (coalesced; neighbouring waves
+// read those too, from cache), so a chunk sees the 64 positions on either
+// side of it.  Group boundaries are one ballot per 64 positions, so a group
+// that starts and ends inside a chunk's 192 positions is found with bit
+// scans and ranked against its members in an LDS copy of them, with no
+// dependent global loads.  A pair whose group reaches past them takes the
 // general path: galloping search for the bounds, members read from global
 // memory (O(m) compares).  A group longer than kSplitGroupMax is listed once,
-// by its first pair, for k_splitLong.
+// by its first pair, for k_splitLong.  All of a wave's loads are issued
+// before any of its chunks is ranked.
 // PACKED: kf holds packed words and vals is unused; the full key of every
 // pair outside a one-pair group is gathered from `low` (a one-pair group
-// needs no compare).
-template <bool PACKED>
+// needs no compare), each word at most once per wave.
+template <bool PACKED, int C>
 __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ kf, const uint32_t *__restrict__ vals,
                                                    const uint32_t *__restrict__ low,
                                                    uint32_t n, uint32_t *__restrict__ valsOut,
                                                    uint64_t *__restrict__ keysOut, uint32_t *__restrict__ ctrl,
                                                    uint2 *__restrict__ list, uint32_t listCap) {
-  __shared__ uint64_t s_k[4][192];  // per wave: keys of positions P0 - 64 .. P0 + 127
-  __shared__ uint32_t s_v[4][192];
+  constexpr int W = C + 2;  // windows of 64 positions: one before, the wave's C chunks, one after
+  __shared__ uint64_t s_k[4][64 * W];  // per wave: keys of positions P0 - 64 .. P0 + 64 (C + 1) - 1
+  __shared__ uint32_t s_v[4][64 * W];
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t P0 = blockIdx.x * 256 + (threadIdx.x & ~63u);  // the wave's first position
-  if (P0 >= n) return;                                          // wave-uniform
+  const uint32_t P0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64u * C;  // the wave's first position
+  if (P0 >= n) return;                                                   // wave-uniform
   // top bits given to positions below 0 and from n on (never a key's: keys have 63 bits)
   constexpr uint64_t kBelow = 1ull << 62, kAbove = 1ull << 61;
-  uint64_t k[3], h[3];
-  uint32_t v[3];
+  uint64_t k[W], h[W];
+  uint32_t v[W];
 #pragma unroll
-  for (int w = 0; w < 3; ++w) {
+  for (int w = 0; w < W; ++w) {
     const int64_t pos = (int64_t)P0 + 64 * (w - 1) + lane;
     const bool in = pos >= 0 && pos < (int64_t)n;
     k[w] = in ? kf[pos] : 0ull;
@@ -124,9 +130,9 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
   }
   // bit l of b[w]: a group starts at position P0 + 64 (w - 1) + l.  Whether
   // one starts at P0 - 64 is left open (0) unless that is position 0.
-  uint64_t b[3];
+  uint64_t b[W];
 #pragma unroll
-  for (int w = 0; w < 3; ++w) {
+  for (int w = 0; w < W; ++w) {
     // every lane takes part in the shuffle (no cross-lane read inside a
     // branch); lane 0 then takes the last top bits of the window before
     const uint64_t up = __shfl_up(h[w], 1, 64);
@@ -134,83 +140,97 @@ __global__ __launch_bounds__(256) void k_splitRank(const uint64_t *__restrict__ 
     b[w] = __ballot(h[w] != (lane == 0 ? before : up));
   }
   if (PACKED) {
-    // Full keys where a compare can read them: the wave's own positions
-    // outside one-pair groups, and the members of the groups that reach
-    // from the wave's first position back into the window before and from
-    // its last position into the window after.
-    const bool s0 = (b[1] >> lane) & 1ull;
-    const bool s1 = lane < 63 ? ((b[1] >> (lane + 1)) & 1ull) : (b[2] & 1ull) != 0;
-    if (P0 + lane < n && !(s0 && s1)) k[1] = packedKey(k[1], low);
-    if (!(b[1] & 1ull) && b[0]) {  // the first group starts in the window before, at its last start
-      const uint32_t from = 63 - __clzll((long long)b[0]);
-      if (lane >= from) k[0] = packedKey(k[0], low);  // positions P0 - 64 + lane >= 0 here
+    // Full keys where a compare can read them.  For chunk c (window c + 1):
+    // its positions outside one-pair groups, and the members of the groups
+    // that reach from its first position back into window c and from its
+    // last position into window c + 2.  The masks of all chunks are joined
+    // first, so each word is gathered once.
+    uint64_t need[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) need[w] = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const uint64_t bp = b[c], bo = b[c + 1], bn = b[c + 2];
+      need[c + 1] |= ~(bo & ((bo >> 1) | (bn << 63)));  // not a start followed by a start
+      if (!(bo & 1ull) && bp) need[c] |= ~0ull << (63 - __clzll((long long)bp));
+      if (!(bn & 1ull)) need[c + 2] |= bn ? (1ull << (__ffsll((long long)bn) - 1)) - 1 : ~0ull;
     }
-    if (!(b[2] & 1ull)) {  // the last group reaches into the window after, up to its first start
-      const uint32_t upto = b[2] ? (uint32_t)__ffsll((long long)b[2]) - 1 : 64u;
-      if (lane < upto && (uint64_t)P0 + 64 + lane < n) k[2] = packedKey(k[2], low);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const int64_t pos = (int64_t)P0 + 64 * (w - 1) + lane;
+      if (pos >= 0 && pos < (int64_t)n && ((need[w] >> lane) & 1ull)) k[w] = packedKey(k[w], low);
     }
   }
-  const bool valid = P0 + lane < n;
-  const uint64_t key = k[1];
-  const uint32_t val = v[1];
-  // own group [gs, ge) in window coordinates (0 = position P0 - 64); -1: not inside
-  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // bits 0..lane
-  int gs = -1, ge = -1;
-  if (b[1] & le)
-    gs = 64 + (63 - __clzll((long long)(b[1] & le)));
-  else if (b[0])
-    gs = 63 - __clzll((long long)b[0]);
-  if (b[1] & ~le)
-    ge = 64 + __ffsll((long long)(b[1] & ~le)) - 1;
-  else if (b[2])
-    ge = 128 + __ffsll((long long)b[2]) - 1;
-  else if ((uint64_t)P0 + 128 >= n)
-    ge = (int)(n - (P0 - 64));  // the group runs to the end of the array
-  const bool local = valid && gs >= 0 && ge >= 0;
-
-  // each lane walks its own group's members in the wave's LDS copy of the
-  // 192 positions (lanes of one group read the same words: broadcasts)
   uint64_t *wk = s_k[threadIdx.x >> 6];
   uint32_t *wv = s_v[threadIdx.x >> 6];
 #pragma unroll
-  for (int w = 0; w < 3; ++w) {
+  for (int w = 0; w < W; ++w) {
     wk[64 * w + lane] = k[w];
     wv[64 * w + lane] = v[w];
   }
-  const int lenMax = __builtin_amdgcn_readfirstlane(waveMaxInt(local ? ge - gs : 0));
-  uint32_t rank = 0;
-  for (int j = 0; j < lenMax; ++j) {
-    const int q = gs + j;
-    if (local && q < ge) {
-      const uint64_t kq = wk[q];
-      const uint32_t vq = wv[q];
-      rank += (kq < key || (kq == key && vq < val)) ? 1u : 0u;
-    }
-  }
-  if (!valid) return;
-  const uint32_t p = P0 + lane;
-  uint32_t start;
-  if (local) {
-    start = P0 - 64 + (uint32_t)gs;
-  } else {
-    const uint64_t hi = hiOf(key);
-    const uint32_t g0 = groupStart(kf, p, hi), g1 = groupEnd(kf, n, p, hi);
-    if (g1 - g0 > kSplitGroupMax) {
-      if (p == g0) {
-        const uint32_t slot = atomicAdd(ctrl + 0, 1u);
-        if (slot < listCap)
-          list[slot] = make_uint2(g0, g1 - g0);
-        else
-          atomicOr(ctrl + 2, kErrList);
+  const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // bits 0..lane
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const uint64_t bp = b[c], bo = b[c + 1], bn = b[c + 2];
+    const uint32_t Pc = P0 + 64u * c;  // the chunk's first position
+    const bool valid = Pc + lane < n;
+    const uint64_t key = k[c + 1];
+    const uint32_t val = v[c + 1];
+    // own group [gs, ge) in chunk coordinates (0 = position Pc - 64); -1: not inside
+    int gs = -1, ge = -1;
+    if (bo & le)
+      gs = 64 + (63 - __clzll((long long)(bo & le)));
+    else if (bp)
+      gs = 63 - __clzll((long long)bp);
+    if (bo & ~le)
+      ge = 64 + __ffsll((long long)(bo & ~le)) - 1;
+    else if (bn)
+      ge = 128 + __ffsll((long long)bn) - 1;
+    else if ((uint64_t)Pc + 128 >= n)
+      ge = (int)(n - (Pc - 64));  // the group runs to the end of the array
+    const bool local = valid && gs >= 0 && ge >= 0;
+    // each lane walks its own group's members (lanes of one group read the
+    // same words: broadcasts)
+    const int lenMax = __builtin_amdgcn_readfirstlane(waveMaxInt(local ? ge - gs : 0));
+    uint32_t rank = 0;
+    for (int j = 0; j < lenMax; ++j) {
+      const int q = gs + j;
+      if (local && q < ge) {
+        const uint64_t kq = wk[64 * c + q];
+        const uint32_t vq = wv[64 * c + q];
+        rank += (kq < key || (kq == key && vq < val)) ? 1u : 0u;
       }
-      return;
     }
-    start = g0;
-    rank = rankGlobal<PACKED>(kf, vals, low, g0, g1, key, val);
+    if (valid) {
+      const uint32_t p = Pc + lane;
+      uint32_t start = 0;
+      bool place = true;
+      if (local) {
+        start = Pc - 64 + (uint32_t)gs;
+      } else {
+        const uint64_t hi = hiOf(key);
+        const uint32_t g0 = groupStart(kf, p, hi), g1 = groupEnd(kf, n, p, hi);
+        if (g1 - g0 > kSplitGroupMax) {
+          if (p == g0) {
+            const uint32_t slot = atomicAdd(ctrl + 0, 1u);
+            if (slot < listCap)
+              list[slot] = make_uint2(g0, g1 - g0);
+            else
+              atomicOr(ctrl + 2, kErrList);
+          }
+          place = false;
+        } else {
+          start = g0;
+          rank = rankGlobal<PACKED>(kf, vals, low, g0, g1, key, val);
+        }
+      }
+      if (place) {
+        valsOut[start + rank] = val;
+        // PACKED: `key` is already the full key unless the group has one pair
+        if (keysOut) keysOut[start + rank] = PACKED ? ((key & ~kPackMask) | low[val]) : key;
+      }
+    }
   }
-  valsOut[start + rank] = val;
-  // PACKED: `key` is already the full key unless the group has one pair
-  if (keysOut) keysOut[start + rank] = PACKED ? ((key & ~kPackMask) | low[val]) : key;
 }
 
 // Long groups: one workgroup per group, a stable LSD over the low 24 bits
@@ -358,12 +378,13 @@ int splitRun(const SplitPlan &p, uint64_t *keys[2], uint32_t *vals[2], uint32_t 
   uint2 *list = reinterpret_cast<uint2 *>(scratch + p.listOff);
   const uint32_t *low = packed ? splitLow(p, scratch) : nullptr;
   uint64_t *keysOut = keepKeys ? keys[r ^ 1] : nullptr;
+  const uint32_t rankBlocks = divUp(p.n, 256 * kSplitChunks);
   if (packed)
-    k_splitRank<true><<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], nullptr, low, p.n, vals[r ^ 1], keysOut, ctrl,
-                                                             list, p.listCap);
+    k_splitRank<true, kSplitChunks><<<rankBlocks, 256, 0, stream()>>>(keys[r], nullptr, low, p.n, vals[r ^ 1],
+                                                                      keysOut, ctrl, list, p.listCap);
   else
-    k_splitRank<false><<<divUp(p.n, 256), 256, 0, stream()>>>(keys[r], vals[r], nullptr, p.n, vals[r ^ 1], keysOut,
-                                                              ctrl, list, p.listCap);
+    k_splitRank<false, kSplitChunks><<<rankBlocks, 256, 0, stream()>>>(keys[r], vals[r], nullptr, p.n, vals[r ^ 1],
+                                                                       keysOut, ctrl, list, p.listCap);
   if (!checkLaunch("k_splitRank")) return -1;
   if (afterStage) afterStage("k_splitRank");
   // long groups are rare (none at the 4.24M-triangle scene): a small grid
