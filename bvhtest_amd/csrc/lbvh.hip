@@ -224,21 +224,63 @@ __device__ __forceinline__ void mortonMapping(const float box[6], bvht_vec3 &off
   sc = {__fdiv_rn(1.0f, box[3] + off.x), __fdiv_rn(1.0f, box[4] + off.y), __fdiv_rn(1.0f, box[5] + off.z)};
 }
 
+// Box of all vertices.  The vertex array is read as 16-byte words (float4
+// q holds floats 4q .. 4q + 3, so its component k is axis (q + k) mod 3)
+// when it is 16-byte aligned: the grid has a multiple of 3 threads, so every
+// word a thread reads has the same phase q mod 3 and the thread keeps three
+// min/max slots (components 0 and 3 in slot 0), turned into axes at the end.
+// The last 3 nv mod 4 floats are read one by one.  An unaligned array is
+// read one vertex at a time.
 __global__ __launch_bounds__(kBoxBlock) void k_vertBox(const bvht_vec3 *__restrict__ vert, uint32_t nv,
                                                        float *__restrict__ partials, uint32_t *__restrict__ zeroPtr,
                                                        size_t zeroWords) {
   const uint32_t tid = threadIdx.x;
   for (size_t i = (size_t)blockIdx.x * kBoxBlock + tid; i < zeroWords; i += (size_t)gridDim.x * kBoxBlock) zeroPtr[i] = 0;
   float mn[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, mx[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+  const uint32_t g = blockIdx.x * kBoxBlock + tid, T = gridDim.x * kBoxBlock;
+  if ((reinterpret_cast<uintptr_t>(vert) & 15u) == 0 && T % 3u == 0) {
+    const float4 *q4 = reinterpret_cast<const float4 *>(vert);
+    const uint64_t nf = 3ull * nv, nq = nf / 4;
+    float sn[3] = {HUGE_VALF, HUGE_VALF, HUGE_VALF}, sx[3] = {-HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
 #pragma unroll 4
-  for (uint32_t i = blockIdx.x * kBoxBlock + tid; i < nv; i += gridDim.x * kBoxBlock) {
-    const bvht_vec3 p = vert[i];
-    mn[0] = fminf(mn[0], p.x);
-    mn[1] = fminf(mn[1], p.y);
-    mn[2] = fminf(mn[2], p.z);
-    mx[0] = fmaxf(mx[0], p.x);
-    mx[1] = fmaxf(mx[1], p.y);
-    mx[2] = fmaxf(mx[2], p.z);
+    for (uint64_t q = g; q < nq; q += T) {
+      const float4 v = q4[q];
+      sn[0] = fminf(sn[0], fminf(v.x, v.w));
+      sx[0] = fmaxf(sx[0], fmaxf(v.x, v.w));
+      sn[1] = fminf(sn[1], v.y);
+      sx[1] = fmaxf(sx[1], v.y);
+      sn[2] = fminf(sn[2], v.z);
+      sx[2] = fmaxf(sx[2], v.z);
+    }
+    const uint32_t ph = g % 3u;  // slot s holds axis (ph + s) mod 3
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t sl = ((uint32_t)a + 3u - ph) % 3u;
+      mn[a] = sl == 0 ? sn[0] : (sl == 1 ? sn[1] : sn[2]);
+      mx[a] = sl == 0 ? sx[0] : (sl == 1 ? sx[1] : sx[2]);
+    }
+    if (g < nf - 4 * nq) {  // at most 3 floats past the last whole word
+      const uint64_t fi = 4 * nq + g;
+      const float x = reinterpret_cast<const float *>(vert)[fi];
+      const uint32_t a = (uint32_t)(fi % 3);
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        if ((uint32_t)b == a) {
+          mn[b] = fminf(mn[b], x);
+          mx[b] = fmaxf(mx[b], x);
+        }
+    }
+  } else {
+#pragma unroll 4
+    for (uint32_t i = g; i < nv; i += T) {
+      const bvht_vec3 p = vert[i];
+      mn[0] = fminf(mn[0], p.x);
+      mn[1] = fminf(mn[1], p.y);
+      mn[2] = fminf(mn[2], p.z);
+      mx[0] = fmaxf(mx[0], p.x);
+      mx[1] = fmaxf(mx[1], p.y);
+      mx[2] = fmaxf(mx[2], p.z);
+    }
   }
   shiftedBox(mn, mx);
   writeBoxPartial(mn, mx, partials);
@@ -553,11 +595,13 @@ uint32_t boxBlocks(uint32_t n) {
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
-// The vertex-box pass: two workgroups per CU.
+// The vertex-box pass: up to two workgroups per CU, a multiple of 3 of them
+// (k_vertBox's 16-byte reads keep one phase per thread).
 uint32_t vertBlocks(uint32_t nv) {
   uint32_t b = divUp((uint64_t)nv + 1, kBoxBlock);
-  uint32_t cap = std::min<uint32_t>(kBoxBlocksMax, (uint32_t)numCUs() * 2);
-  return b < 1 ? 1 : (b > cap ? cap : b);
+  const uint32_t cap = std::min<uint32_t>(kBoxBlocksMax, (uint32_t)numCUs() * 2) / 3 * 3;
+  b = divUp(b, 3) * 3;
+  return b > cap ? cap : b;
 }
 
 // Every Morton workgroup ends with one global atomic per non-empty bin of
