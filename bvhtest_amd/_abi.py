@@ -182,7 +182,7 @@ def load_library(path=LIB_PATH):
     """
     if not os.path.exists(path):
         raise RuntimeError(f"bvhtest_amd HIP library not built: {path} is missing "
-                           f"(run __graft_entry__.build() or make -C bvhtest_amd/csrc)")
+                           f"(build it with make -C bvhtest_amd/csrc)")
     lib = ctypes.CDLL(path)
     missing = []
     for name, (res, args) in SIGNATURES.items():

@@ -1,6 +1,6 @@
 """Learned removal priority on a held-out 16.9M-triangle scene.
 
-The north-star configuration of the learned priority: the shipped 12-64-64-1
+The evaluation setting of the learned priority: the shipped 12-64-64-1
 BF16 MLP (bvhtest_amd/data/priority_mlp.npy, trained on records of scenes
 with seeds 100-131) against the plain Bittner13 heuristic (Bittner13CUDA.cu:
 480-500) on a scene it never saw (seed 200: 8,192 icospheres + 2,000,000 soup
