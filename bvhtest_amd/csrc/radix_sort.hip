@@ -9,6 +9,11 @@ namespace {
 constexpr uint32_t kFlagAgg = 1u << 30;  // tile aggregate published
 constexpr uint32_t kFlagInc = 2u << 30;  // inclusive prefix published
 constexpr uint32_t kCountMask = (1u << 30) - 1;
+// The wide pass's status words: two digits' 15-bit tile counts per word,
+// bit 15 and bit 31 set when published.
+constexpr uint32_t kHalfFlags = 0x80008000u, kHalfCount = 0x7FFFu;
+constexpr uint32_t kWideStatusWords = kSortRadix / 2;  // per tile
+static_assert(kWideTile <= kHalfCount, "a tile's count of one digit fits 15 bits");
 constexpr uint32_t kLookWindow = 8;  // look-back statuses loaded per round trip
 
 // Histogram of all passes: LDS counters per block, then one global atomic
@@ -221,8 +226,10 @@ __global__ __launch_bounds__(kSortBlock, 3) void k_onesweep(const K *__restrict_
 //    first pass: values are the input index, made when they are staged) or
 //    kValsNone (a keys-only sort).
 //  - The tile's global offsets are the sums of every earlier tile's
-//    published per-digit counts, loaded by all 1024 threads at once (16-byte
-//    loads of four digits, 8 in flight each) instead of a chained look-back.
+//    published per-digit counts, loaded by all 1024 threads at once instead
+//    of a chained look-back.  The counts are published two digits to a word
+//    (15 bits each), so a 16-byte load brings eight digits and, with 8 loads
+//    in flight per thread, one round trip covers 256 predecessors.
 //    Tile ids come from `counter`, so every tile a workgroup waits for has
 //    started; the wait for a count is bounded and raises the error flag.
 enum { kValsLoaded = 0, kValsIdentity = 1, kValsNone = 2 };
@@ -292,7 +299,11 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
       s_wave[w][tid] = total;
       total += c;
     }
-    storeRelaxedAgent(status + (size_t)tile * kSortRadix + tid, kFlagAgg | total);
+    // two digits per word: 15-bit counts (a tile holds at most kWideTile
+    // keys), each half flagged when published
+    const uint32_t odd = __shfl_xor(total, 1, 64);  // waves 0..3, all lanes
+    if ((tid & 1u) == 0)
+      storeRelaxedAgent(status + (size_t)tile * kWideStatusWords + tid / 2, kHalfFlags | total | (odd << 16));
   }
   const uint32_t hbase = blockExclusiveScan<kWideBlock>(tid < kSortRadix ? hist[tid] : 0u, s_scan);
   const uint32_t bstart = blockExclusiveScan<kWideBlock>(total, s_scan);
@@ -333,44 +344,54 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   __syncthreads();  // s_wave is free from here: it holds the prefix partials
 
   // offsets: the counts of every earlier tile.  Thread (g, q) sums digits
-  // 4q .. 4q + 3 over predecessors g, g + 16, ... with 16-byte coherent
-  // loads, 8 in flight (two round trips for 243 predecessors); the 16
-  // partial sums of a digit are then added by its owner.
+  // 8q .. 8q + 7 over predecessors g, g + 32, ... with 16-byte coherent
+  // loads, 8 in flight (one round trip for up to 256 predecessors); the 32
+  // partial sums of a digit are then added in two steps.
   {
-    constexpr uint32_t QUADS = kSortRadix / 4, QG = kWideBlock / QUADS, QB = 8;
-    const uint32_t q = tid % QUADS, g = tid / QUADS;
-    const BufferRsrc st = bufferRsrc(status, tile * kSortRadix * 4u);  // the earlier tiles' words
-    uint32_t sum[4] = {0u, 0u, 0u, 0u}, spins = 0;
+    constexpr uint32_t OCTS = kSortRadix / 8, QG = kWideBlock / OCTS, QB = 8;
+    static_assert(QG == 2 * WAVES, "two look-back groups per row of s_wave");
+    const uint32_t q = tid % OCTS, g = tid / OCTS;
+    const BufferRsrc st = bufferRsrc(status, tile * kWideStatusWords * 4u);  // the earlier tiles' words
+    uint32_t sum[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}, spins = 0;
     for (uint32_t j0 = g; j0 < tile; j0 += QG * QB) {
       BufferU32x4 sv[QB];
 #pragma unroll
       for (uint32_t b = 0; b < QB; ++b) {
         const uint32_t j = j0 + b * QG;
-        sv[b] = j < tile ? bufferLoadU32x4Coherent(st, (j * kSortRadix + 4 * q) * 4u)
-                         : BufferU32x4{kFlagAgg, kFlagAgg, kFlagAgg, kFlagAgg};
+        sv[b] = j < tile ? bufferLoadU32x4Coherent(st, (j * kWideStatusWords + 4 * q) * 4u)
+                         : BufferU32x4{kHalfFlags, kHalfFlags, kHalfFlags, kHalfFlags};
       }
 #pragma unroll
       for (uint32_t b = 0; b < QB; ++b) {
         const uint32_t j = j0 + b * QG;
         BufferU32x4 v = sv[b];
         // not yet published (j < tile: tile 0 never waits); the wait is bounded
-        while (((v[0] & ~kCountMask) == 0 || (v[1] & ~kCountMask) == 0 || (v[2] & ~kCountMask) == 0 ||
-                (v[3] & ~kCountMask) == 0)) {
+        while (((v[0] & v[1] & v[2] & v[3]) & kHalfFlags) != kHalfFlags) {
           if (++spins > (1u << 24)) {
             atomicOr(err, 1u);
-            v = BufferU32x4{kFlagAgg, kFlagAgg, kFlagAgg, kFlagAgg};
+            v = BufferU32x4{kHalfFlags, kHalfFlags, kHalfFlags, kHalfFlags};
             break;
           }
           __builtin_amdgcn_s_sleep(1);
           memoryChanged();
-          v = bufferLoadU32x4Coherent(st, (j * kSortRadix + 4 * q) * 4u);
+          v = bufferLoadU32x4Coherent(st, (j * kWideStatusWords + 4 * q) * 4u);
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sum[c] += v[c] & kCountMask;
+        for (int c = 0; c < 4; ++c) {
+          sum[2 * c] += v[c] & kHalfCount;
+          sum[2 * c + 1] += (v[c] >> 16) & kHalfCount;
+        }
       }
     }
+    if (g >= WAVES) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s_wave[g][4 * q + c] = sum[c];
+      for (int c = 0; c < 8; ++c) s_wave[g - WAVES][8 * q + c] = sum[c];
+    }
+    __syncthreads();
+    if (g < WAVES) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) s_wave[g][8 * q + c] += sum[c];
+    }
   }
   __syncthreads();
   if (tid < kSortRadix) {
