@@ -463,6 +463,7 @@ __global__ __launch_bounds__(kWideBlock, 8) void k_segOnesweepWide(
   const SegDesc sd = segs[tt.x];
   const uint32_t lt = tt.y, segStart = sd.faceBase, segN = sd.n;
   const uint32_t first = lt * TILE + wave * 64 * ITEMS + lane;  // mesh-local index of item 0
+  const uint32_t meshPrefix = tid < R ? prefix[(size_t)tt.x * passes * R + tid] : 0u;  // issued now, used last
   const BufferRsrc keys = bufferRsrc(keysIn, total * 4u);
   uint32_t kw[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -558,7 +559,7 @@ __global__ __launch_bounds__(kWideBlock, 8) void k_segOnesweepWide(
   }
   if (tid < R) {
     if (lt > 0) storeRelaxedAgent(status + (size_t)g * R + tid, kSegFlagInc | (excl + count));
-    s_gstart[tid] = segStart + prefix[(size_t)tt.x * passes * R + tid] + excl;
+    s_gstart[tid] = segStart + meshPrefix + excl;
   }
   __syncthreads();
 

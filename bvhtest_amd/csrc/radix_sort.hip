@@ -258,6 +258,7 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t first = tile * TILE + wave * 64 * ITEMS + lane;  // item i is key first + 64 i
+  const uint32_t histDigit = tid < kSortRadix ? hist[tid] : 0u;  // issued now, used after the ranking
 
   const BufferRsrc keys = bufferRsrc(keysIn, n * (uint32_t)sizeof(K));
   const uint32_t dw = (WIDE && shift >= 32) ? 1u : 0u;  // the digit's word
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(kWideBlock, 1) void k_onesweepWide(const K *__restr
     if ((tid & 1u) == 0)
       storeRelaxedAgent(status + (size_t)tile * kWideStatusWords + tid / 2, kHalfFlags | total | (odd << 16));
   }
-  const uint32_t hbase = blockExclusiveScan<kWideBlock>(tid < kSortRadix ? hist[tid] : 0u, s_scan);
+  const uint32_t hbase = blockExclusiveScan<kWideBlock>(histDigit, s_scan);
   const uint32_t bstart = blockExclusiveScan<kWideBlock>(total, s_scan);
   if (tid < kSortRadix) s_bstart[tid] = bstart;
   __syncthreads();
