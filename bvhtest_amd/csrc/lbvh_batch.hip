@@ -862,6 +862,7 @@ bvht_LBVH_BatchWorkingMemory bvht_LBVH_allocateBatch(uint32_t numMeshes, const u
       !devAlloc(&in->vals[1], in->totalFaces, "batch values") ||
       !devAlloc(&in->rec, 3ull * in->totalFaces, "batch leaf boxes") ||
       !devAlloc(&in->sortScratch, 16 + in->histWords + in->statusWords, "batch sort") ||
+      !fillU32(in->sortScratch, 0u, 16) ||  // the error word reads 0 before the first build
       !check(hipStreamCreateWithFlags(&in->capStream, hipStreamNonBlocking), "batch capture stream")) {
     bvht_LBVH_freeBatch(&m);
     return m;

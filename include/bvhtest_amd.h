@@ -315,9 +315,9 @@ void bvht_LBVH_freeBatch(bvht_LBVH_BatchWorkingMemory *mem);
 bool bvht_LBVH_batchOffsets(const bvht_LBVH_BatchWorkingMemory *mem, uint32_t i, uint32_t *faceOffset,
                             uint32_t *vertOffset, uint32_t *nodeOffset);
 /* Error bits of the last batched sort (extension, no counterpart in the
- * reference; synchronises): 0 = the sort ran to completion, 1 = a look-back
- * wait gave up (the trees of that build are not valid).  0xFFFFFFFF on a
- * bad argument or a failed copy. */
+ * reference; synchronises): 0 = the sort ran to completion (also before the
+ * first build), 1 = a look-back wait gave up (the trees of that build are
+ * not valid).  0xFFFFFFFF on a bad argument or a failed copy. */
 uint32_t bvht_LBVH_batchStatus(const bvht_LBVH_BatchWorkingMemory *mem);
 /* Autotuner of the batched build: times every tile size 2^8..2^11 with both
  * digit widths (8 and 10 bits) on the caller's input and keeps the fastest

@@ -26,6 +26,7 @@ def test_batch_segments_bit_exact(bvh, orc, tile, bits):
     v, f, fc, vc = bvh.scene_batch(24, 77, 1, 20000)
     print(f"tile={tile} bits={bits} meshes={len(fc)} faces={len(f)} min={fc.min()} max={fc.max()}", flush=True)
     b = bvh.BatchBuilder(v, f, fc, vc, tile_log2=tile, radix_bits=bits)
+    assert b.status() == 0  # before the first build
     b.build(use_graph=False)
     check_segments(bvh, orc, b, v, f, fc, vc, range(len(fc)))
 
