@@ -933,6 +933,8 @@ bvht_LBVH_WorkingMemory bvht_LBVH_allocateWorkingMemoryEx(const bvht_MeshRaw *me
   }
   if (!check(hipMemsetAsync(m.params, 0, sizeof(BuildParams), stream()), "LBVH params init") ||
       !check(hipMemsetAsync(m.atomicLocks, 0, sizeof(uint32_t) * m.numLocks, stream()), "LBVH locks init") ||
+      // the sort's counters and error words read 0 before the first build
+      !check(hipMemsetAsync(m.sortScratch, 0, sizeof(uint32_t) * sortZeroWords(in), stream()), "LBVH sort init") ||
       !check(hipStreamSynchronize(stream()), "LBVH init sync")) {
     LBVH_freeWorkingMemory(&m);
     return m;

@@ -247,8 +247,9 @@ void LBVH_doCUDASyc(void);
  * (digitBits 16 uses the triangles' box directly). */
 bool bvht_LBVH_build(bvht_LBVH_WorkingMemory *mem, const bvht_MeshRaw *mesh, bvht_CUDAMemoryBVHPointer *bvh);
 /* Error bits of the last sort (extension, synchronises): 0 = the sort ran
- * to completion.  MSD sort: 1 = histogram does not match the keys, 2 = a
- * bucket ran out of key bits, 4 = a piece exceeded the in-LDS limit;
+ * to completion (also before the first build).  MSD sort: 1 = histogram
+ * does not match the keys, 2 = a bucket ran out of key bits, 4 = a piece
+ * exceeded the in-LDS limit;
  * onesweep: 1 = a look-back wait gave up; 63-bit keys also 8 = the list of
  * long groups of equal top bits overflowed.
  * 0xFFFFFFFF on a bad argument or a failed copy. */

@@ -52,6 +52,7 @@ def test_fused_build_bit_exact(bvh, orc, cases, name, bits, digits):
     keys, sf, expect, scene = orc.lbvh(v, f, bits)
     mesh = bvh.DeviceMesh(v, f)
     b = bvh.LBVHBuilder(mesh, bits, digit_bits=digits)
+    assert b.sort_status() == 0  # before the first build
     tree = b.allocate_bvh()
     b.build(tree)
     got_sf = b.sorted_faces()
