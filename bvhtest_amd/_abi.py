@@ -113,6 +113,7 @@ SIGNATURES = {
     "bvht_LBVH_sortStatus": (c_u32, [P(LBVHWorkingMemory)]),
     "bvht_sortKeyIndexPairs": (c_bool, [c_vp, c_u32, c_u32, c_u32, c_vp, P(c_u32)]),
     "bvht_LBVH_setTiming": (None, [c_bool]),
+    "bvht_LBVH_setTimingPassGroups": (None, [c_bool]),
     "bvht_LBVH_buildNoAlloc": (c_bool, [P(MeshRaw), P(CUDAMemoryBVHPointer), c_u32]),
     "bvht_LBVH_stageTimes": (ctypes.c_int, [P(c_f), ctypes.c_int]),
     "bvht_LBVH_stageName": (ctypes.c_char_p, [ctypes.c_int]),

@@ -556,6 +556,8 @@ Internal *internalOf(bvht_LBVH_WorkingMemory *m) { return reinterpret_cast<Inter
 struct Timing {
   static constexpr int kMax = 32;
   bool enabled = false;
+  bool groupPasses = false;  // the sort's digit passes as one entry (no events between them)
+  int lastPass = -1;         // index of the sort's last digit pass
   hipEvent_t ev[kMax] = {};
   const char *name[kMax] = {};
   bool created = false;
@@ -570,6 +572,10 @@ struct Timing {
     (void)hipEventRecord(ev[recorded], stream());
     name[recorded] = label;
     recorded++;
+  }
+  void recordPass(int pass) {
+    if (groupPasses && pass != lastPass) return;
+    record("k_onesweep");
   }
 };
 Timing &timing() {
@@ -692,10 +698,11 @@ bool runSort(bvht_LBVH_WorkingMemory *mem) {
     mem->sortedKeys = keepKeys ? mem->keys[0] : nullptr;
     return true;
   }
+  tm.lastPass = (in->useSplit ? in->split.plan.passes : in->plan.passes) - 1;
   if (in->useSplit) {
     uint64_t *k2[2] = {(uint64_t *)mem->keys[0], (uint64_t *)mem->keys[1]};
     const int r = splitRun(in->split, k2, mem->values, (uint32_t *)mem->sortScratch, keepKeys, in->packedKeys,
-                           tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr,
+                           tm.enabled ? [](int pass) { timing().recordPass(pass); } : nullptr,
                            tm.enabled ? [](const char *s) { timing().record(s); } : nullptr);
     if (r < 0) return false;
     in->resultBuf = r;
@@ -704,7 +711,7 @@ bool runSort(bvht_LBVH_WorkingMemory *mem) {
     return true;
   }
   int r = sortRun(in->plan, mem->keys, mem->values, true, keepKeys, (uint32_t *)mem->sortScratch,
-                  tm.enabled ? [](int) { timing().record("k_onesweep"); } : nullptr);
+                  tm.enabled ? [](int pass) { timing().recordPass(pass); } : nullptr);
   if (r < 0) return false;
   in->resultBuf = r;
   mem->sortedValues = mem->values[r];
@@ -1083,6 +1090,7 @@ uint32_t bvht_LBVH_sortStatus(bvht_LBVH_WorkingMemory *mem) {
 }
 
 void bvht_LBVH_setTiming(bool enable) { timing().enabled = enable; }
+void bvht_LBVH_setTimingPassGroups(bool group) { timing().groupPasses = group; }
 
 int bvht_LBVH_stageTimes(float *ms, int maxStages) {
   Timing &tm = timing();

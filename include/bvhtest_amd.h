@@ -279,6 +279,10 @@ bool bvht_sortKeyIndexPairs(const void *keys, uint32_t n, uint32_t keyBits, uint
  * k_buildLevelUp).
  * Returns the number of entries written. */
 void bvht_LBVH_setTiming(bool enable);
+/* With group on, the onesweep digit passes of a timed build are one entry,
+ * k_onesweep, covering all of them (no events between the passes, so no
+ * event overhead inside; divide by the pass count for a mean pass). */
+void bvht_LBVH_setTimingPassGroups(bool group);
 int bvht_LBVH_stageTimes(float *ms, int maxStages);
 const char *bvht_LBVH_stageName(int i);
 
