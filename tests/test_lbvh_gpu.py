@@ -231,3 +231,43 @@ def test_errors_are_reported(bvh):
         import ctypes
         if not L.LBVH_allocateBVH(ctypes.byref(tree.ptr), ctypes.byref(mesh.raw)):
             bvh._fail("LBVH_allocateBVH")
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_stage_timing_entries(bvh, bits):
+    """Per-kernel timing of a build names every kernel in launch order; with
+    pass groups the onesweep passes are one entry.  The timed builds give
+    the same tree as an untimed one."""
+    import ctypes
+
+    L = bvh.lib()
+    v, f = soup(50000, seed=7)
+    mesh = bvh.DeviceMesh(v, f)
+    b = bvh.LBVHBuilder(mesh, bits)
+    tree = b.allocate_bvh()
+    b.build(tree)
+    ref = tree.download().tobytes()
+
+    def timed(group):
+        L.bvht_LBVH_setTiming(True)
+        L.bvht_LBVH_setTimingPassGroups(group)
+        try:
+            b.build(tree)
+            buf = (ctypes.c_float * 32)()
+            k = L.bvht_LBVH_stageTimes(buf, 32)
+            return [L.bvht_LBVH_stageName(i).decode() for i in range(k)], [buf[i] for i in range(k)]
+        finally:
+            L.bvht_LBVH_setTimingPassGroups(False)
+            L.bvht_LBVH_setTiming(False)
+
+    passes = 5 if bits == 63 else 4
+    names, ms = timed(False)
+    assert names[:3] == ["k_vertBox", "k_morton", "k_mortonFix"]
+    assert names.count("k_onesweep") == passes and "k_buildLevel0" in names
+    assert all(x > 0 for x in ms)
+    assert tree.download().tobytes() == ref
+    gnames, gms = timed(True)
+    assert gnames.count("k_onesweep") == 1
+    assert [x for x in gnames if x != "k_onesweep"] == [x for x in names if x != "k_onesweep"]
+    assert all(x > 0 for x in gms)
+    assert tree.download().tobytes() == ref
