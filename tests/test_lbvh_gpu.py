@@ -145,6 +145,29 @@ def test_full_size_63bit_scene(bvh, orc):
         tree.free()
 
 
+@pytest.mark.timeout(300)
+def test_9M_triangles_both_key_widths_bit_exact(bvh, orc):
+    """4,096 spheres + 2,000,000 soup triangles (about 9.1M, past 2^23): more
+    than twice the keys one round of wide sort tiles holds, so both key
+    widths sort with the 256-thread passes over several rounds of tiles
+    (30-bit keys past one round are tested nowhere else); permutation and
+    every node byte against the oracle."""
+    v, f = bvh.scene_spheres_soup(4096, 2000000, 5)
+    print(f"scene n={len(f)}", flush=True)
+    assert len(f) > (1 << 23)
+    mesh = bvh.DeviceMesh(v, f)
+    for bits in (30, 63):
+        keys, sf, expect, _ = orc.lbvh(v, f, bits)
+        b = bvh.LBVHBuilder(mesh, bits)
+        tree = b.allocate_bvh()
+        b.build(tree)
+        assert b.sort_status() == 0
+        assert np.array_equal(b.sorted_faces(), sf), bits
+        assert tree.download().tobytes() == expect.tobytes(), bits
+        tree.free()
+        b.free()
+
+
 def test_63bit_past_wide_tiles(bvh, orc):
     """2,048 spheres + 800,000 soup triangles (4.54M): more keys than one round
     of wide sort tiles holds (256 x 17,408 on MI355X), so the full build
